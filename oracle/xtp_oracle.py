@@ -1,0 +1,363 @@
+"""FP64 CPU restatement of the VOTCA-XTP GW-BSE hot path (the oracle).
+
+TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg may import this module, and only as the checker / CPU baseline.
+The product path (votca_amd, libxtpgpu.so) never imports or calls it.
+
+Every function cites the reference file:line whose loop it restates. Layout follows
+the reference: an Mmn tensor is a list of mtotal slices, each (ntotal x P); here it is
+one numpy array M[m, n, P] (m = slice index, n = row, P = aux column), so that
+M[m] is exactly the reference's Mmn_[m] (xtp/include/votca/xtp/threecenter.h:125).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+HRT2EV = 27.21138602  # tools/include/votca/tools/constants.h:53
+
+
+# ---------------------------------------------------------------- three-centre
+def pseudo_invsqrt_gwbse(V: np.ndarray, S: np.ndarray, etol: float = 5e-7) -> np.ndarray:
+    """((S^-1/2 V S^-1/2)^-1/2 S^-1/2)^T -- xtp/src/libxtp/aomatrices/aomatrix.cc:53-86."""
+    ev, U = np.linalg.eigh(S)
+    d = np.where(ev < etol, 0.0, 1.0 / np.sqrt(np.maximum(ev, etol)))
+    Ssqrt = (U * d) @ U.T
+    ortho = Ssqrt @ V @ Ssqrt
+    ev2, U2 = np.linalg.eigh(ortho)
+    d2 = np.where(ev2 < etol, 0.0, 1.0 / np.sqrt(np.maximum(ev2, etol)))
+    Vm1 = (U2 * d2) @ U2.T
+    return (Vm1 @ Ssqrt).T
+
+
+def fill_mmn(T3c: np.ndarray, C: np.ndarray, mmin: int, mmax: int, nmin: int, nmax: int,
+             Saux: np.ndarray, Vaux: np.ndarray) -> np.ndarray:
+    """TCMatrix_gwbse::Fill -- xtp/src/libxtp/threecenter.cc:72-90 with Fill3cMO
+    (xtp/src/libxtp/libint2_calls.cc:595-651): M[m](n,Q) = sum_{mu,nu} C[mu,n] (Q|mu nu) C[nu,m],
+    then right-multiplied by the aux inverse square root."""
+    Cm = C[:, mmin:mmax + 1]
+    Cn = C[:, nmin:nmax + 1]
+    M = np.einsum("qab,an,bm->mnq", T3c, Cn, Cm, optimize=True)
+    return multiply_right(M, pseudo_invsqrt_gwbse(Vaux, Saux, 5e-7))
+
+
+def multiply_right(M: np.ndarray, A: np.ndarray) -> np.ndarray:
+    """TCMatrix_gwbse::MultiplyRightWithAuxMatrix -- xtp/src/libxtp/threecenter.cc:54-65."""
+    return np.einsum("mnp,pq->mnq", M, A, optimize=True)
+
+
+# ---------------------------------------------------------------- RPA
+def rpa_update_energies(dft: np.ndarray, gw: np.ndarray, qpmin: int, homo: int,
+                        rpamin: int, rpamax: int) -> np.ndarray:
+    """RPA::UpdateRPAInputEnergies / ShiftUncorrectedEnergies / getMaxCorrection --
+    xtp/src/libxtp/gwbse/rpa.cc:32-73."""
+    rpatotal = rpamax - rpamin + 1
+    e = dft[rpamin:rpamin + rpatotal].copy()
+    gwsize = len(gw)
+    e[qpmin - rpamin:qpmin - rpamin + gwsize] = gw
+    lumo = homo + 1
+    qpmax = qpmin + gwsize - 1
+
+    def maxcorr(lo, hi):
+        r = hi - lo + 1
+        return float(np.max(np.abs(e[lo:lo + r] - dft[lo - rpamin:lo - rpamin + r])))
+
+    occ = maxcorr(qpmin, homo)
+    virt = maxcorr(lumo, qpmax)
+    e[:qpmin] -= occ
+    if rpamax - qpmax > 0:
+        e[len(e) - (rpamax - qpmax):] += virt
+    return e
+
+
+def rpa_weights(energies: np.ndarray, homo: int, rpamin: int, rpamax: int,
+                freq, kind: str, eta: float = 1e-4) -> np.ndarray:
+    """Per-transition weight d[v, c] of eps = 1 + sum_v M_v^T diag(d_v) M_v.
+
+    kind 'imag'    : 4 D/(D^2+w^2)                       rpa.cc:117-118
+    kind 'real'    : 2[(D-w)/((D-w)^2+eta^2) + (D+w)/((D+w)^2+eta^2)]   rpa.cc:119-125
+    kind 'complex' : -2 chi, chi as rpa.cc:182-190 (sign folded in: rpa.cc:197)
+    """
+    lumo = homo + 1
+    n_occ = lumo - rpamin
+    n_unocc = rpamax - lumo + 1
+    D = energies[n_occ:n_occ + n_unocc][None, :] - energies[:n_occ][:, None]
+    if kind == "imag":
+        w = float(freq)
+        return 4.0 * D / (D * D + w * w)
+    if kind == "real":
+        w = float(freq)
+        a = D - w
+        b = D + w
+        return 2.0 * (a / (a * a + eta * eta) + b / (b * b + eta * eta))
+    if kind == "complex":
+        wr, wi = complex(freq).real, complex(freq).imag
+        dm = wr - D
+        dp = wr + D
+        s1 = (wi + eta) ** 2
+        s2 = (wi - eta) ** 2
+        chi = dm / (dm * dm + s1) - dp / (dp * dp + s2)
+        return -2.0 * chi
+    raise ValueError(kind)
+
+
+def rpa_epsilon(M: np.ndarray, energies: np.ndarray, homo: int, rpamin: int, rpamax: int,
+                freq, kind: str, eta: float = 1e-4) -> np.ndarray:
+    """RPA::calculate_epsilon<imag> / calculate_epsilon_r(complex) --
+    xtp/src/libxtp/gwbse/rpa.cc:75-140 and 145-202, with OpenMP_CUDA::A_TDA
+    (xtp/src/libxtp/openmp_cuda.cc:236-256): eps = 1 + sum_v M_v^T diag(d_v) M_v with
+    M_v = Mmn_[v].bottomRows(n_unocc)."""
+    lumo = homo + 1
+    n_occ = lumo - rpamin
+    n_unocc = rpamax - lumo + 1
+    d = rpa_weights(energies, homo, rpamin, rpamax, freq, kind, eta)
+    P = M.shape[2]
+    eps = np.zeros((P, P))
+    for v in range(n_occ):
+        Mv = M[v][M.shape[1] - n_unocc:, :]
+        eps += Mv.T @ (d[v][:, None] * Mv)
+    eps[np.diag_indices(P)] += 1.0
+    return eps
+
+
+# ---------------------------------------------------------------- BSE operator
+class BSEOperatorOracle:
+    """BSE_OPERATOR<cqp,cx,cd,cd2> -- xtp/src/libxtp/gwbse/bse_operator.cc:28-188,
+    xtp/include/votca/xtp/bse_operator.h:40-87, with the OpenMP_CUDA CPU branches
+    (xtp/src/libxtp/openmp_cuda.cc:259-478). Compound index I(v,c) = ctotal*v + c
+    (xtp/include/votca/xtp/vc2index.h:36-54)."""
+
+    def __init__(self, cqp, cx, cd, cd2, eps_inv, M, Hqp, homo, rpamin, qpmin, vmin, cmax):
+        assert not (cd != 0 and cd2 != 0)
+        self.c = (cqp, cx, cd, cd2)
+        self.eps_inv = np.asarray(eps_inv, dtype=np.float64)
+        self.M = M
+        self.Hqp = Hqp
+        self.homo, self.rpamin, self.qpmin, self.vmin, self.cmax = homo, rpamin, qpmin, vmin, cmax
+        self.bse_vmax = homo
+        self.bse_cmin = homo + 1
+        self.vtotal = self.bse_vmax - vmin + 1
+        self.ctotal = cmax - self.bse_cmin + 1
+        self.size = self.vtotal * self.ctotal
+
+    def I(self, v, c):
+        return self.ctotal * v + c
+
+    def hqp_row(self, v1, c1):
+        """BSE_OPERATOR::Hqp_row -- bse_operator.cc:121-132."""
+        cqp = self.c[0]
+        R = np.zeros((self.ctotal, self.vtotal))
+        cmin = self.vtotal
+        R[:, v1] += cqp * self.Hqp[cmin:cmin + self.ctotal, c1 + cmin]
+        R[c1, :] -= cqp * self.Hqp[:self.vtotal, v1]
+        return R.reshape(-1, order="F")
+
+    def matmul(self, X: np.ndarray) -> np.ndarray:
+        """BSE_OPERATOR::matmul -- bse_operator.cc:39-119 (row construction then Hx blocks)."""
+        cqp, cx, cd, cd2 = self.c
+        X = np.asarray(X, dtype=np.float64)
+        vmin = self.vmin - self.rpamin
+        cmin = self.bse_cmin - self.rpamin
+        ct, vt = self.ctotal, self.vtotal
+        out = np.zeros((self.size, X.shape[1]))
+        ei = self.eps_inv
+        for c1 in range(ct):
+            if cd != 0:
+                T = -cd * self.M[c1 + cmin][cmin:cmin + ct, :] * ei[None, :]
+            elif cd2 != 0:
+                T = -cd2 * self.M[c1 + cmin][vmin:vmin + vt, :] * ei[None, :]
+            for v1 in range(vt):
+                row = np.zeros(self.size)
+                if cd != 0:
+                    mat = self.M[v1 + vmin][vmin:vmin + vt, :]
+                    row += (T @ mat.T).reshape(-1, order="F")  # (ct x vt), c fast
+                if cd2 != 0:
+                    mat = self.M[v1 + vmin][cmin:cmin + ct, :]
+                    row += (mat @ T.T).reshape(-1, order="F")  # (ct x vt)
+                if cqp != 0:
+                    row += self.hqp_row(v1, c1)
+                out[self.I(v1, c1)] = row @ X
+        if cx > 0:
+            for v1 in range(vt):
+                M1 = cx * self.M[v1 + vmin][cmin:cmin + ct, :]
+                for v2 in range(v1, vt):
+                    B = M1 @ self.M[v2 + vmin][cmin:cmin + ct, :].T
+                    out[v1 * ct:(v1 + 1) * ct] += B @ X[v2 * ct:(v2 + 1) * ct]
+                    if v1 != v2:
+                        out[v2 * ct:(v2 + 1) * ct] += B.T @ X[v1 * ct:(v1 + 1) * ct]
+        return out
+
+    def diagonal(self) -> np.ndarray:
+        """BSE_OPERATOR::diagonal -- bse_operator.cc:134-175."""
+        cqp, cx, cd, cd2 = self.c
+        vmin = self.vmin - self.rpamin
+        cmin = self.bse_cmin - self.rpamin
+        ct, vt = self.ctotal, self.vtotal
+        ei = self.eps_inv
+        res = np.zeros(self.size)
+        for v in range(vt):
+            for c in range(ct):
+                e = 0.0
+                if cx != 0:
+                    e += cx * float(np.sum(self.M[v + vmin][cmin + c] ** 2))
+                if cqp != 0:
+                    cq = vt
+                    e += cqp * (self.Hqp[c + cq, c + cq] - self.Hqp[v, v])
+                if cd != 0:
+                    e -= cd * float(np.sum(self.M[c + cmin][c + cmin] * ei * self.M[v + vmin][v + vmin]))
+                if cd2 != 0:
+                    e -= cd2 * float(np.sum(self.M[c + cmin][v + vmin] * ei * self.M[v + vmin][c + cmin]))
+                res[self.I(v, c)] = e
+        return res
+
+
+def bse_dense(cqp, cx, cd, cd2, eps_inv, M, Hqp, homo, rpamin, vmin, cmax) -> np.ndarray:
+    """Explicit BSE matrix from the closed-form matrix elements the reference's
+    matmul contracts (bse_operator.cc:61-116): H[(v,c),(v',c')] =
+      cqp (d_vv' Hqp_cc' - d_cc' Hqp_v'v) + cx sum_P M^P_vc M^P_v'c'
+      - cd sum_P eps^-1_P M^P_cc' M^P_vv' - cd2 sum_P eps^-1_P M^P_cv' M^P_vc'.
+    Index independent of the row loop, so it pins the loop restatement above."""
+    vm = vmin - rpamin
+    cm = homo + 1 - rpamin
+    vt = homo - vmin + 1
+    ct = cmax - homo
+    ei = np.asarray(eps_inv)
+    Mvc = M[vm:vm + vt, cm:cm + ct, :]          # M^P_{v c}
+    Mcc = M[cm:cm + ct, cm:cm + ct, :]          # M^P_{c c'}
+    Mvv = M[vm:vm + vt, vm:vm + vt, :]          # M^P_{v v'}
+    Mcv = M[cm:cm + ct, vm:vm + vt, :]          # M^P_{c v'}
+    H = np.zeros((vt, ct, vt, ct))
+    if cx:
+        H += cx * np.einsum("acp,bdp->acbd", Mvc, Mvc)
+    if cd:
+        H -= cd * np.einsum("cdp,abp,p->acbd", Mcc, Mvv, ei)
+    if cd2:
+        H -= cd2 * np.einsum("cbp,adp,p->acbd", Mcv, Mvc, ei)
+    if cqp:
+        Hc = Hqp[vt:vt + ct, vt:vt + ct]
+        Hv = Hqp[:vt, :vt]
+        H += cqp * np.einsum("ab,cd->acbd", np.eye(vt), Hc)
+        H -= cqp * np.einsum("ba,cd->acbd", Hv, np.eye(ct))
+    return H.reshape(vt * ct, vt * ct)
+
+
+# ---------------------------------------------------------------- Davidson
+class DavidsonOracle:
+    """DavidsonSolver -- xtp/include/votca/xtp/davidsonsolver.h:64-280 and
+    xtp/src/libxtp/davidsonsolver.cc:29-558 (SYMM matrix type: Ritz, DPR/Olsen,
+    'min'/'safe'/'max' size update, CGS twice, restart)."""
+
+    TOL = {"loose": 1e-3, "normal": 1e-4, "strict": 1e-5, "lapack": 1e-9}
+
+    def __init__(self, tol=1e-4, correction="DPR", update="safe", iter_max=50,
+                 max_search_space=0):
+        self.tol = self.TOL.get(tol, tol) if isinstance(tol, str) else float(tol)
+        self.correction = correction
+        self.update = update
+        self.iter_max = iter_max
+        self.max_search_space = max_search_space
+        self.info_converged = False
+        self.num_iterations = 0
+
+    def size_update(self, neigen):  # davidsonsolver.cc:149-170
+        if self.update == "min":
+            return neigen
+        if self.update == "safe":
+            return int(1.5 * neigen) if neigen < 20 else neigen + 10
+        return 2 * neigen
+
+    @staticmethod
+    def gramschmidt(Q, nstart):  # davidsonsolver.cc:456-497
+        nupdate = Q.shape[1] - nstart
+        norms = np.linalg.norm(Q[:, nstart:], axis=0)
+        for rep in range(2):
+            if nstart > 0:
+                Q[:, nstart:] -= Q[:, :nstart] @ (Q[:, :nstart].T @ Q[:, nstart:])
+                Q[:, nstart:] /= np.linalg.norm(Q[:, nstart:], axis=0)
+            for j in range(nstart + 1, Q.shape[1]):
+                Qb = Q[:, nstart:j]
+                Q[:, j] -= Qb @ (Qb.T @ Q[:, j])
+                nrm = np.linalg.norm(Q[:, j])
+                if rep == 1 and nrm <= 1e-12 * norms[j - nstart]:
+                    raise RuntimeError("Linear dependencies in Gram-Schmidt.")
+                Q[:, j] /= nrm
+        return Q
+
+    def solve(self, matmul, diag, neigen):
+        diag = np.asarray(diag, dtype=np.float64)
+        n = diag.size
+        if self.max_search_space < neigen:
+            self.max_search_space = neigen * 5
+        if self.max_search_space > n:
+            self.max_search_space = n
+        size_guess = 2 * neigen
+        restart_size = size_guess
+        idx = np.argsort(diag, kind="stable")
+        V = np.zeros((n, size_guess))
+        for j in range(size_guess):
+            V[idx[j], j] = 1.0
+        size_update = self.size_update(neigen)
+        AV = None
+        T = None
+        self.info_converged = False
+        for it in range(self.iter_max):
+            self.num_iterations = it
+            if it == 0:
+                AV = matmul(V)
+                T = V.T @ AV
+            else:
+                old = AV.shape[1]
+                new = V[:, old:]
+                AVn = matmul(new)
+                AV = np.hstack([AV, AVn])
+                Tn = np.zeros((V.shape[1], V.shape[1]))
+                Tn[:old, :old] = T
+                Tn[:, old:] = V.T @ AVn
+                Tn[old:, :old] = Tn[:old, old:].T
+                T = Tn
+            lam, U = np.linalg.eigh(T)
+            need = min(T.shape[1], max(restart_size, size_update))
+            lam = lam[:need]
+            U = U[:, :need]
+            q = V @ U
+            res = AV @ U - q * lam[None, :]
+            rn = np.linalg.norm(res, axis=0)
+            conv = rn[:size_update] < self.tol
+            if np.all(conv[:neigen]):
+                self.info_converged = True
+                break
+            if it == self.iter_max - 1:
+                lam = lam.copy()
+                q = q.copy()
+                for i in range(neigen):
+                    if not conv[i]:
+                        lam[i] = 0.0
+                        q[:, i] = 0.0
+                break
+            newv = []
+            for j in range(size_update):
+                if conv[j]:
+                    continue
+                if self.correction == "DPR":
+                    w = -res[:, j] / (diag - lam[j])
+                else:
+                    d = -res[:, j] / (diag - lam[j])
+                    xq = q[:, j]
+                    num = -xq @ d
+                    den = -xq @ (-xq / (diag - lam[j]))
+                    w = d + (num / den) * xq
+                w = np.where(np.isfinite(w), w, 0.0)
+                newv.append(w / np.linalg.norm(w))
+            nup = len(newv)
+            V = np.hstack([V, np.array(newv).T])
+            V = self.gramschmidt(V, V.shape[1] - nup)
+            if V.shape[1] > self.max_search_space:  # restart, davidsonsolver.cc:499-522
+                newV = np.hstack([q[:, :restart_size], V[:, -nup:]])
+                AV = AV @ U[:, :restart_size]
+                T = newV[:, :restart_size].T @ AV
+                V = newV
+        self.eigenvalues = lam[:neigen]
+        vecs = q[:, :neigen]
+        nrm = np.linalg.norm(vecs, axis=0)
+        nrm[nrm == 0] = 1.0
+        self.eigenvectors = vecs / nrm
+        return self.eigenvalues, self.eigenvectors
