@@ -1,0 +1,171 @@
+/*
+ * xtpgpu -- C-ABI of the MI355X (gfx950) GW-BSE dense-contraction hot path.
+ *
+ * Every entry point returns an int status (XTPGPU_OK = 0, negative on error) and never
+ * lets a C++ exception cross the boundary; xtpgpu_last_error() returns the message of
+ * the last failure on the calling thread. Host pointers are borrowed for the duration of
+ * a call only (copy-in / copy-out); device state lives in opaque handles that own their
+ * memory until the matching *_destroy. Matrices at the boundary are column-major FP64
+ * with explicit leading dimensions and 64-bit sizes, as Eigen::MatrixXd in the reference
+ * (tools/include/votca/tools/types.h). Index conventions follow the reference:
+ *   - an Mmn tensor is mtotal slices, each (ntotal x auxsize), slice m / row n / column P
+ *     (xtp/include/votca/xtp/threecenter.h:70-142);
+ *   - the BSE compound index is I(v,c) = ctotal*v + c (xtp/include/votca/xtp/vc2index.h:36-54).
+ * Calls on one context are stream-ordered; use one context per calling thread.
+ */
+#ifndef XTPGPU_H
+#define XTPGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  XTPGPU_OK = 0,
+  XTPGPU_ERR_HIP = -1,            /* HIP runtime / library failure (reference: std::runtime_error, cudamatrix.cc:25-37) */
+  XTPGPU_ERR_OOM = -2,            /* device memory request exceeds free memory (cudamatrix.cc:97-113) */
+  XTPGPU_ERR_SHAPE = -3,          /* inconsistent sizes / indices */
+  XTPGPU_ERR_NOT_CONVERGED = -4,  /* iterative solver did not converge (davidsonsolver.cc:524-547) */
+  XTPGPU_ERR_ARG = -5,            /* invalid argument */
+  XTPGPU_ERR_NUMERIC = -6         /* numerical failure, e.g. Gram-Schmidt linear dependence (davidsonsolver.cc:490-494) */
+};
+
+/* Arithmetic of a contraction. FP64: plain FP64 MFMA (the drop-in form).
+ * BF16: BF16 operands, FP32 accumulation. BF16X3: operands split a = a_hi + a_lo in
+ * BF16, products hi*hi + hi*lo + lo*hi accumulated in FP32. */
+enum { XTPGPU_FP64 = 0, XTPGPU_BF16 = 1, XTPGPU_BF16X3 = 3 };
+
+/* RPA frequency kinds: RPA::calculate_epsilon_i (rpa.h:47-49), calculate_epsilon_r(double)
+ * (rpa.h:51-53), calculate_epsilon_r(std::complex<double>) (rpa.cc:145-202). */
+enum { XTPGPU_EPS_IMAG = 0, XTPGPU_EPS_REAL = 1, XTPGPU_EPS_COMPLEX = 2 };
+
+typedef struct xtpgpu_ctx_s* xtpgpu_ctx;
+typedef struct xtpgpu_mmn_s* xtpgpu_mmn;
+typedef struct xtpgpu_bseop_s* xtpgpu_bseop;
+
+/* ---------------------------------------------------------------- errors, devices */
+const char* xtpgpu_last_error(void);
+int xtpgpu_device_count(int* count);
+/* One context per (device, stream); replaces OpenMP_CUDA::SetNoGPUs / GPU_data
+ * (xtp/src/libxtp/openmp_cuda.cc:28-71). flags must be 0. */
+int xtpgpu_ctx_create(int device, unsigned flags, xtpgpu_ctx* out);
+int xtpgpu_ctx_destroy(xtpgpu_ctx ctx);
+int xtpgpu_ctx_synchronize(xtpgpu_ctx ctx);
+/* Live kernel timing: when enabled, HIP events are recorded on the context's stream around
+ * the dominant launches (names: "rpa_eps_gemm", "hd_stage1", "hd_stage2", "hd2_stage1",
+ * "hd2_stage2", "hx", "hqp", "davidson_ortho"), together with each launch's algorithmic
+ * FLOPs and bytes. xtpgpu_ctx_timer synchronizes the stream and returns the totals. */
+int xtpgpu_ctx_set_profiling(xtpgpu_ctx ctx, int enable);
+int xtpgpu_ctx_timer(xtpgpu_ctx ctx, const char* name, double* total_ms, int64_t* launches,
+                     double* flops, double* bytes);
+int xtpgpu_ctx_reset_timers(xtpgpu_ctx ctx);
+/* Stream-ordered device copies of host memory, for callers that keep data resident. */
+int xtpgpu_malloc(xtpgpu_ctx ctx, int64_t bytes, void** dptr);
+int xtpgpu_free(xtpgpu_ctx ctx, void* dptr);
+int xtpgpu_memcpy_h2d(xtpgpu_ctx ctx, void* dst, const void* src, int64_t bytes);
+int xtpgpu_memcpy_d2h(xtpgpu_ctx ctx, void* dst, const void* src, int64_t bytes);
+
+/* ---------------------------------------------------------------- TCMatrix_gwbse */
+/* TCMatrix_gwbse::Initialize (threecenter.cc:29-48): zero tensor, slices m in [mmin,mmax],
+ * rows n in [nmin,nmax], auxsize columns. */
+int xtpgpu_mmn_create(xtpgpu_ctx ctx, int64_t mmin, int64_t mmax, int64_t nmin, int64_t nmax,
+                      int64_t auxsize, xtpgpu_mmn* out);
+int xtpgpu_mmn_destroy(xtpgpu_mmn mmn);
+/* Copy one slice Mmn_[m] (ntotal x auxsize, column-major, leading dim ld) in / out
+ * (TCMatrix_gwbse::operator[], threecenter.h:101-104). m is relative to mmin. */
+int xtpgpu_mmn_set_slice(xtpgpu_mmn mmn, int64_t m, const double* slice, int64_t ld);
+int xtpgpu_mmn_get_slice(xtpgpu_mmn mmn, int64_t m, double* slice, int64_t ld);
+/* TCMatrix_gwbse::MultiplyRightWithAuxMatrix (threecenter.cc:54-65): Mmn_[m] <- Mmn_[m] * A
+ * for every slice; A is auxsize x auxsize column-major on the host. */
+int xtpgpu_mmn_multiply_right(xtpgpu_mmn mmn, const double* A, int64_t lda);
+/* TCMatrix_gwbse::Fill3cMO AO->MO step (libint2_calls.cc:595-651): from the AO three-centre
+ * integrals (Q|mu nu) (auxsize x nbf x nbf, element (Q,mu,nu) at Q*nbf*nbf + nu*nbf + mu)
+ * and MO coefficients C (nbf x nmo column-major), sets Mmn_[m](n,Q) = sum C(mu,n)(Q|mu nu)C(nu,m)
+ * for the tensor's m/n windows. The aux metric is applied afterwards with
+ * xtpgpu_mmn_multiply_right, as TCMatrix_gwbse::Fill does (threecenter.cc:72-90). */
+int xtpgpu_mmn_fill_ao2mo(xtpgpu_mmn mmn, const double* ao3c, int64_t nbf, const double* C,
+                          int64_t ldc, int64_t nmo);
+/* Synthetic tensor for benchmarks: M^P_{mn} = sigma_P * g(seed, P, min(m,n), max(m,n)),
+ * sigma_P = sigma0 * exp(-decay * P / auxsize), g a counter-based standard normal variate
+ * (splitmix64 hash of (seed,P,i,j), Irwin-Hall sum of 12 uniforms: integer arithmetic, so a
+ * host restatement reproduces it bit for bit); symmetric in (m,n) as a three-centre quantity is. */
+int xtpgpu_mmn_fill_synthetic(xtpgpu_mmn mmn, uint64_t seed, double sigma0, double decay);
+int xtpgpu_mmn_sizes(xtpgpu_mmn mmn, int64_t* mtotal, int64_t* ntotal, int64_t* auxsize);
+
+/* ---------------------------------------------------------------- RPA */
+/* RPA::calculate_epsilon<imag> and calculate_epsilon_r(complex) (rpa.cc:75-202):
+ * eps(w) = 1 + sum_{v<n_occ} M_v^T diag(d_v(w)) M_v, M_v = Mmn_[v].bottomRows(n_unocc),
+ * for n_omega frequencies in one pass. energies: rpatotal RPA input energies
+ * (RPA::setRPAInputEnergies). out: auxsize x auxsize x n_omega column-major (host).
+ * eta: RPA broadening (the reference fixes 1e-4, rpa.h:90). */
+int xtpgpu_rpa_epsilon(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* energies, int64_t homo,
+                       int64_t rpamin, int64_t rpamax, int kind, const double* omega_re,
+                       const double* omega_im, int n_omega, double eta, int precision,
+                       double* out);
+/* Same, result left on the device (out_dev: auxsize*auxsize*n_omega doubles). */
+int xtpgpu_rpa_epsilon_device(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* energies, int64_t homo,
+                              int64_t rpamin, int64_t rpamax, int kind, const double* omega_re,
+                              const double* omega_im, int n_omega, double eta, int precision,
+                              double* out_dev);
+
+/* ---------------------------------------------------------------- BSE operator */
+/* BSE_OPERATOR<cqp,cx,cd,cd2>(eps0inv, Mmn, Hqp) + configure(BSEOperator_Options)
+ * (bse_operator.h:32-48, bse_operator.cc:28-37). Hqp is the BSE-sized (vtotal+ctotal)^2 QP
+ * Hamiltonian (BSE::AdjustHqpSize, bse.cc:147-186). The device operator packs the Mmn blocks
+ * it contracts (c c', v v', v c) into K-contiguous layouts once; Mmn must not change while the
+ * operator lives (the reference holds a const reference with the same rule, bse_operator.h:73-75).
+ * precision_mask: bit (1<<XTPGPU_FP64) always; add (1<<XTPGPU_BF16)|(1<<XTPGPU_BF16X3) to
+ * also keep BF16 shadows for the mixed-precision forms. */
+int xtpgpu_bse_create(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* eps0inv, const double* Hqp,
+                      int64_t ldhqp, int64_t homo, int64_t rpamin, int64_t qpmin, int64_t vmin,
+                      int64_t cmax, int cqp, int cx, int cd, int cd2, unsigned precision_mask,
+                      xtpgpu_bseop* out);
+int xtpgpu_bse_destroy(xtpgpu_bseop op);
+int xtpgpu_bse_size(xtpgpu_bseop op, int64_t* size);
+/* BSE_OPERATOR::diagonal (bse_operator.cc:134-175). */
+int xtpgpu_bse_diagonal(xtpgpu_bseop op, double* out);
+/* BSE_OPERATOR::matmul (bse_operator.cc:39-119): Y = H X for k columns, host buffers. */
+int xtpgpu_bse_matmul(xtpgpu_bseop op, int precision, const double* X, int64_t ldx, int64_t k,
+                      double* Y, int64_t ldy);
+/* Same on device buffers (X, Y: size x k, leading dims ldx/ldy). */
+int xtpgpu_bse_matmul_device(xtpgpu_bseop op, int precision, const double* X, int64_t ldx,
+                             int64_t k, double* Y, int64_t ldy);
+
+/* ---------------------------------------------------------------- Davidson */
+typedef struct {
+  double tolerance;        /* residual-norm tolerance (DavidsonSolver::set_tolerance, davidsonsolver.cc:122-134) */
+  int correction;          /* 0 = DPR, 1 = OLSEN (davidsonsolver.cc:105-115) */
+  int size_update;         /* 0 = min, 1 = safe, 2 = max (davidsonsolver.cc:136-170) */
+  int iter_max;            /* default 50 */
+  int max_search_space;    /* reference BSE uses 10*nmax (bse.cc:280) */
+  int precision;           /* operator arithmetic during the iterations */
+  /* Mixed-precision refinement (precision != FP64): after the iterations converge to
+   * `tolerance` with the mixed operator, up to refine_max rounds of FP64 Rayleigh-Ritz on
+   * the Ritz vectors plus DPR corrections, until the FP64 residual norm < refine_tolerance. */
+  double refine_tolerance;
+  int refine_max;
+} xtpgpu_davidson_opts;
+
+typedef struct {
+  int converged;           /* 1 if all neigen roots met the tolerance */
+  int iterations;          /* DavidsonSolver::num_iterations */
+  int matvec_columns;      /* operator columns applied in `precision` */
+  int matvec_columns_fp64; /* FP64 operator columns applied during refinement */
+  int refine_rounds;
+  double max_residual;     /* final max residual norm over the neigen roots (FP64 operator when refined) */
+  double seconds;          /* wall time of the solve */
+} xtpgpu_davidson_info;
+
+/* DavidsonSolver::solve(A, neigen) for the SYMM matrix type (davidsonsolver.h:64-122) on a
+ * device BSE operator (TDA). evals: neigen; evecs: size x neigen column-major (host, may be
+ * null). Non-converged roots are zeroed like storeNotConvergedData (davidsonsolver.cc:530-547)
+ * and XTPGPU_ERR_NOT_CONVERGED is returned. */
+int xtpgpu_davidson_solve(xtpgpu_bseop op, int64_t neigen, const xtpgpu_davidson_opts* opts,
+                          double* evals, double* evecs, int64_t ldv, xtpgpu_davidson_info* info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XTPGPU_H */

@@ -11,6 +11,8 @@ M[m] is exactly the reference's Mmn_[m] (xtp/include/votca/xtp/threecenter.h:125
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 HRT2EV = 27.21138602  # tools/include/votca/tools/constants.h:53
@@ -361,3 +363,56 @@ class DavidsonOracle:
         nrm[nrm == 0] = 1.0
         self.eigenvectors = vecs / nrm
         return self.eigenvalues, self.eigenvectors
+
+
+# ---------------------------------------------------------------- synthetic inputs
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def _splitmix64(x: np.ndarray) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        x = (x + np.uint64(0x9E3779B97F4A7C15))
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return x ^ (x >> np.uint64(31))
+
+
+def synth_normal(seed: int, p, i, j) -> np.ndarray:
+    """Host twin of the device generator (votca_amd/csrc/kernels_misc.h synth_normal):
+    splitmix64 hash of (seed, P, i, j), Irwin-Hall sum of 12 uniform 16-bit integers."""
+    p = np.asarray(p, dtype=np.uint64)
+    i = np.asarray(i, dtype=np.uint64)
+    j = np.asarray(j, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        h = _splitmix64(np.uint64(seed) + np.zeros(np.broadcast(p, i, j).shape, dtype=np.uint64))
+        h = _splitmix64(h ^ (p * np.uint64(0x100000001B3)))
+        h = _splitmix64(h ^ (i * np.uint64(0xC2B2AE3D27D4EB4F)))
+        h = _splitmix64(h ^ (j * np.uint64(0x165667B19E3779F9)))
+        s = np.zeros(h.shape, dtype=np.uint64)
+        f = np.uint64(0xFFFF)
+        for _ in range(3):
+            h = _splitmix64(h)
+            s += (h & f) + ((h >> np.uint64(16)) & f) + ((h >> np.uint64(32)) & f) + ((h >> np.uint64(48)) & f)
+    return (s.astype(np.float64) + 6.0) * (1.0 / 65536.0) - 6.0
+
+
+def synthetic_mmn(seed: int, mmin: int, mmax: int, nmin: int, nmax: int, aux: int,
+                  sigma0: float = 0.3, decay: float = 3.0) -> np.ndarray:
+    """M[m, n, P] = sigma_P * synth_normal(seed, P, min(m,n), max(m,n)),
+    sigma_P = sigma0 * exp(-decay * P / aux)."""
+    m = np.arange(mmin, mmax + 1)[:, None, None]
+    n = np.arange(nmin, nmax + 1)[None, :, None]
+    P = np.arange(aux)[None, None, :]
+    a = np.minimum(m, n)
+    b = np.maximum(m, n)
+    # math.exp: the same libm exp the library's host side uses for its sigma table
+    sig = np.array([sigma0 * math.exp(-decay * float(q) / float(aux)) for q in range(aux)])
+    return synth_normal(seed, P, a, b) * sig[None, None, :]
+
+
+def synthetic_energies(seed: int, n_occ: int, n_virt: int):
+    """Occupied levels sorted uniform in [-20,-5] eV, virtual sorted uniform in [0,30] eV, in Ha."""
+    rng = np.random.default_rng(seed)
+    eo = np.sort(rng.uniform(-20.0, -5.0, n_occ))
+    ev = np.sort(rng.uniform(0.0, 30.0, n_virt))
+    return np.concatenate([eo, ev]) / HRT2EV

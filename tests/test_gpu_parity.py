@@ -1,0 +1,205 @@
+"""GPU parity: the HIP path through the C-ABI against the oracle and the reference's
+golden fixtures. FP64 must agree with the oracle to round-off; the mixed forms within the
+tolerances written in each test."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import integrals as I
+from oracle import mmio
+from oracle import xtp_oracle as X
+from votca_amd import pyxtp as px
+
+pytestmark = pytest.mark.gpu
+G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+EPS_INV_BSE_OP = np.array([0.999807798016267, 0.994206065211371, 0.917916768047073,
+    0.902913813951883, 0.902913745974602, 0.902913584797742, 0.853352878674581, 0.853352727016914,
+    0.853352541699637, 0.79703468058566, 0.797034577207669, 0.797034400395582, 0.787701833916331,
+    0.518976361745313, 0.518975064844033, 0.518973712898761, 0.459286057710524])
+
+
+def relerr(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return px.default_context()
+
+
+@pytest.fixture(scope="module")
+def ints():
+    return I.ao_integrals(os.path.join(G, "threecenter_gwbse/molecule.xyz"),
+                          os.path.join(G, "threecenter_gwbse/3-21G.xml"))
+
+
+def dev_mmn(ctx, M, mmin=0, nmin=0):
+    t = px.TCMatrix_gwbse(ctx).Initialize(M.shape[2], mmin, mmin + M.shape[0] - 1, nmin, nmin + M.shape[1] - 1)
+    return t.set_tensor(M)
+
+
+def test_mmn_roundtrip(ctx):
+    rng = np.random.default_rng(0)
+    M = rng.normal(size=(3, 5, 7))
+    t = dev_mmn(ctx, M)
+    assert np.array_equal(t.get_tensor(), M)
+
+
+def test_synthetic_generator_bitexact(ctx):
+    t = px.TCMatrix_gwbse(ctx).Initialize(13, 2, 9, 1, 11)
+    t.fill_synthetic(42, 0.3, 3.0)
+    ref = X.synthetic_mmn(42, 2, 9, 1, 11, 13)
+    assert np.array_equal(t.get_tensor(), ref)
+
+
+def test_multiply_right(ctx):
+    rng = np.random.default_rng(1)
+    M = rng.normal(size=(6, 9, 70))
+    A = rng.normal(size=(70, 70))
+    t = dev_mmn(ctx, M)
+    t.MultiplyRightWithAuxMatrix(A)
+    assert relerr(t.get_tensor(), X.multiply_right(M, A)) < 1e-13
+
+
+def test_fill_ao2mo_matches_threecenter_fixture(ctx, ints):  # test_threecenter_gwbse.cc:36-126
+    S, Saux, V, T = ints
+    C = mmio.read_matrix(os.path.join(G, "threecenter_gwbse/MOs.mm"))
+    t = px.TCMatrix_gwbse(ctx).Initialize(17, 0, 5, 0, 7)
+    t.Fill(T, C, X.pseudo_invsqrt_gwbse(V, Saux))
+    M = t.get_tensor()
+    assert relerr(M, X.fill_mmn(T, C, 0, 5, 0, 7, Saux, V)) < 1e-12
+    for k in (0, 2, 4):
+        assert mmio.isapprox(M[k], mmio.read_matrix(os.path.join(G, f"threecenter_gwbse/ref{k}b.mm")), 1e-5)
+
+
+def test_rpa_fixtures_gpu(ctx, ints):  # test_rpa.cc:69-140
+    S, Saux, V, T = ints
+    C = mmio.read_matrix(os.path.join(G, "rpa/eigenvectors.mm"))
+    e = mmio.read_vector(os.path.join(G, "rpa/eigenvals.mm"))
+    M = X.fill_mmn(T, C, 0, 16, 0, 16, Saux, V)
+    rpa = px.RPA(dev_mmn(ctx, M))
+    rpa.configure(4, 0, 16)
+    rpa.setRPAInputEnergies(e)
+    for name, f, kind in (("i_ref", 0.5, "imag"), ("r_ref", 0.0, "real"), ("r_complex_ref", 0.5 + 0.5j, "complex")):
+        g = rpa.calculate_epsilon_i(f) if kind == "imag" else rpa.calculate_epsilon_r(f)
+        assert mmio.isapprox(g, mmio.read_matrix(os.path.join(G, f"rpa/{name}.mm")), 1e-4), name
+        assert relerr(g, X.rpa_epsilon(M, e, 4, 0, 16, f, kind)) < 1e-13, name
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("bf16x3", 2e-5), ("bf16", 2e-2)])
+def test_rpa_random_multi_frequency(ctx, prec, tol):
+    rng = np.random.default_rng(2)
+    o, v, P = 9, 70, 150
+    M = rng.normal(scale=0.3, size=(o + v, o + v, P))
+    e = np.concatenate([np.sort(rng.uniform(-1, -0.2, o)), np.sort(rng.uniform(0, 1.5, v))])
+    rpa = px.RPA(dev_mmn(ctx, M))
+    rpa.configure(o - 1, 0, o + v - 1)
+    rpa.setRPAInputEnergies(e)
+    freqs = [0.0, 0.3, 1.7, 5.0]
+    g = rpa.calculate_epsilon(px.EPS_IMAG, freqs, prec)
+    for i, f in enumerate(freqs):
+        ref = X.rpa_epsilon(M, e, o - 1, 0, o + v - 1, f, "imag")
+        assert relerr(g[:, :, i], ref) < tol, (prec, f, relerr(g[:, :, i], ref))
+        assert np.array_equal(g[:, :, i], g[:, :, i].T)
+
+
+@pytest.mark.parametrize("name,c", [("hqp", (1, 0, 0, 0)), ("hx", (0, 1, 0, 0)), ("hd", (0, 0, 1, 0)),
+                                    ("hd2", (0, 0, 0, 1))])
+def test_bse_operator_fixtures_gpu(ctx, ints, name, c):  # test_bse_operator.cc:37-151
+    S, Saux, V, T = ints
+    C = mmio.read_matrix(os.path.join(G, "bse_operator/MOs.mm"))
+    Hqp = mmio.read_matrix(os.path.join(G, "bse_operator/Hqp.mm"))
+    t = px.TCMatrix_gwbse(ctx).Initialize(17, 0, 16, 0, 16)
+    t.Fill(T, C, X.pseudo_invsqrt_gwbse(V, Saux))
+    t.MultiplyRightWithAuxMatrix(mmio.read_matrix(os.path.join(G, "bse_operator/rpa_op.mm")))
+    op = px.BSE_OPERATOR(*c, EPS_INV_BSE_OP, t, Hqp).configure(px.BSEOperator_Options(4, 0, 0, 0, 8))
+    H = op.matmul(np.eye(op.size))
+    assert mmio.isapprox(H, mmio.read_matrix(os.path.join(G, f"bse_operator/{name}_ref.mm")), 1e-3)
+    assert mmio.isapprox(np.diag(H), op.diagonal(), 1e-3)
+    ref = X.BSEOperatorOracle(*c, EPS_INV_BSE_OP, t.get_tensor(), Hqp, 4, 0, 0, 0, 8).matmul(np.eye(op.size))
+    assert np.abs(H - ref).max() < 1e-12
+
+
+def random_bse_case(seed, vt=7, ct=33, P=70, rpamin=0, vmin_off=0, offdiag=False):
+    rng = np.random.default_rng(seed)
+    ntot = vmin_off + vt + ct
+    M = rng.normal(scale=0.3, size=(ntot, ntot, P))
+    M = 0.5 * (M + M.transpose(1, 0, 2))
+    eps = rng.uniform(0.1, 1.0, P)
+    hs = vt + ct
+    Hqp = np.diag(np.concatenate([np.sort(rng.uniform(-1, -0.3, vt)), np.sort(rng.uniform(0.1, 2, ct))]))
+    if offdiag:
+        B = 0.01 * rng.normal(size=(hs, hs))
+        Hqp = Hqp + B + B.T
+    homo = rpamin + vmin_off + vt - 1
+    vmin = rpamin + vmin_off
+    cmax = homo + ct
+    return M, eps, Hqp, homo, rpamin, vmin, cmax
+
+
+@pytest.mark.parametrize("c", [(1, 2, 1, 0), (1, 0, 1, 0), (0, 2, 0, 1), (1, 0, 0, 0), (0, 1, 0, 0), (0, 0, 1, 0),
+                               (0, 0, 0, 1), (1, 1, 1, 0), (0, 1, 0, 1)])
+def test_bse_operator_random_fp64(ctx, c):
+    M, eps, Hqp, homo, rpamin, vmin, cmax = random_bse_case(10, vmin_off=2, offdiag=True)
+    t = dev_mmn(ctx, M, mmin=rpamin, nmin=rpamin)
+    op = px.BSE_OPERATOR(*c, eps, t, Hqp).configure(px.BSEOperator_Options(homo, rpamin, 0, vmin, cmax))
+    ref_op = X.BSEOperatorOracle(*c, eps, M, Hqp, homo, rpamin, 0, vmin, cmax)
+    Xv = np.random.default_rng(11).normal(size=(op.size, 5))
+    assert relerr(op.matmul(Xv), ref_op.matmul(Xv)) < 1e-13
+    assert relerr(op.diagonal(), ref_op.diagonal()) < 1e-13
+
+
+@pytest.mark.parametrize("prec,tol", [("bf16x3", 1e-4), ("bf16", 3e-2)])
+@pytest.mark.parametrize("c", [(1, 2, 1, 0), (0, 2, 0, 1)])
+def test_bse_operator_random_mixed(ctx, c, prec, tol):
+    M, eps, Hqp, homo, rpamin, vmin, cmax = random_bse_case(12, vt=11, ct=90, P=130)
+    t = dev_mmn(ctx, M)
+    op = px.BSE_OPERATOR(*c, eps, t, Hqp, precisions=("fp64", prec)).configure(
+        px.BSEOperator_Options(homo, rpamin, 0, vmin, cmax))
+    Xv = np.random.default_rng(13).normal(size=(op.size, 7))
+    ref = op.matmul(Xv, "fp64")
+    assert relerr(op.matmul(Xv, prec), ref) < tol
+
+
+def test_davidson_fp64_matches_dense(ctx):
+    M, eps, Hqp, homo, rpamin, vmin, cmax = random_bse_case(20, vt=8, ct=40, P=60)
+    t = dev_mmn(ctx, M)
+    op = px.SingletOperator_TDA(eps, t, Hqp).configure(px.BSEOperator_Options(homo, rpamin, 0, vmin, cmax))
+    H = X.bse_dense(1, 2, 1, 0, eps, M, Hqp, homo, rpamin, vmin, cmax)
+    w = np.linalg.eigvalsh(H)
+    ds = px.DavidsonSolver()
+    ds.set_tolerance(1e-9)
+    ds.set_max_search_space(50)
+    ev, vec = ds.solve(op, 5)
+    assert ds.converged()
+    assert np.abs(ev - w[:5]).max() < 1e-8
+    assert np.abs(np.linalg.norm(vec, axis=0) - 1).max() < 1e-12
+    r = H @ vec - vec * ev
+    assert np.linalg.norm(r, axis=0).max() < 1e-8
+
+
+@pytest.mark.parametrize("prec", ["bf16", "bf16x3"])
+def test_davidson_mixed_refined_matches_dense(ctx, prec):
+    M, eps, Hqp, homo, rpamin, vmin, cmax = random_bse_case(21, vt=8, ct=40, P=60)
+    t = dev_mmn(ctx, M)
+    op = px.SingletOperator_TDA(eps, t, Hqp, precisions=("fp64", prec)).configure(
+        px.BSEOperator_Options(homo, rpamin, 0, vmin, cmax))
+    w = np.linalg.eigvalsh(X.bse_dense(1, 2, 1, 0, eps, M, Hqp, homo, rpamin, vmin, cmax))
+    ds = px.DavidsonSolver()
+    ds.set_tolerance(1e-4 if prec == "bf16x3" else 1e-2)
+    ds.set_precision(prec, refine_tolerance=1e-7, refine_max=8)
+    ev, _ = ds.solve(op, 5)
+    assert ds.converged(), (ds.info.max_residual, ds.info.refine_rounds)
+    assert np.abs(ev - w[:5]).max() < 1e-9
+
+
+def test_davidson_forced_failure(ctx):  # test_davidson.cc:107-125
+    M, eps, Hqp, homo, rpamin, vmin, cmax = random_bse_case(22, vt=5, ct=30, P=40)
+    t = dev_mmn(ctx, M)
+    op = px.SingletOperator_TDA(eps, t, Hqp).configure(px.BSEOperator_Options(homo, rpamin, 0, vmin, cmax))
+    ds = px.DavidsonSolver()
+    ds.set_tolerance(1e-12)
+    ds.set_iter_max(1)
+    ev, vec = ds.solve(op, 4)
+    assert not ds.converged()

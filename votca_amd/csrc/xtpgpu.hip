@@ -1,0 +1,1000 @@
+// libxtpgpu: MI355X (gfx950) GW-BSE dense-contraction hot path behind a C-ABI.
+//
+// Reference surface replaced here (file:line in the reference tree):
+//   TCMatrix_gwbse             xtp/include/votca/xtp/threecenter.h:70-142, xtp/src/libxtp/threecenter.cc:29-131
+//   RPA::calculate_epsilon*    xtp/src/libxtp/gwbse/rpa.cc:75-202
+//   BSE_OPERATOR::matmul/diag  xtp/src/libxtp/gwbse/bse_operator.cc:28-188
+//   DavidsonSolver             xtp/include/votca/xtp/davidsonsolver.h:64-280, xtp/src/libxtp/davidsonsolver.cc
+//   OpenMP_CUDA / CudaPipeline xtp/src/libxtp/openmp_cuda.cc, xtp/src/libxtp/cudapipeline.cc
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "gemm_kernels.h"
+#include "host_eigen.h"
+#include "kernels_misc.h"
+#include "xtpgpu_internal.h"
+
+using namespace xg;
+
+static thread_local std::string g_last_error;
+
+template <class F>
+static int guarded(F&& f) {
+  try {
+    f();
+    return XTPGPU_OK;
+  } catch (const xg::Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return XTPGPU_ERR_ARG;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return XTPGPU_ERR_ARG;
+  }
+}
+
+namespace {
+
+// ------------------------------------------------------------------ epilogues
+struct EpiStore {  // C[b*bs + m*sm + n*sn] = alpha*v + beta*C
+  double* C;
+  long sm, sn, bs;
+  double alpha, beta;
+  template <typename V>
+  __device__ void operator()(int b, int, int m, int n, V v) const {
+    double* p = C + (long)b * bs + (long)m * sm + (long)n * sn;
+    double r = alpha * (double)v;
+    if (beta != 0.0) r += beta * (*p);
+    *p = r;
+  }
+};
+
+struct EpiPartial {  // split-K partials: part[(ks*batch + b)*M*N + n*M + m]
+  double* part;
+  int M, N, batch;
+  template <typename V>
+  __device__ void operator()(int b, int ks, int m, int n, V v) const {
+    part[((long)ks * batch + b) * (long)M * N + (long)n * M + m] = (double)v;
+  }
+};
+
+struct EpiEpsSym {  // eps[b](m,n) = eps[b](n,m) = v + delta_mn, written once from m >= n
+  double* out;
+  long P;
+  template <typename V>
+  __device__ void operator()(int b, int, int m, int n, V v) const {
+    if (m < n) return;
+    double r = (double)v + (m == n ? 1.0 : 0.0);
+    double* o = out + (long)b * P * P;
+    o[(long)n * P + m] = r;
+    o[(long)m * P + n] = r;
+  }
+};
+
+// Hd / Hd2 first-stage product, stored K-contiguous for the second stage.
+// Row m = p*rows + r, column n = kv*cols + j  ->  S[(r*kvec + kv)*(pc*colsp) + p*colsp + j]
+// (by_row) or S[(kv*rows + r)*(pc*colsp) + p*colsp + j]; pads (j >= cols) stay zero.
+struct EpiStage1 {
+  double* S64;
+  unsigned short* Sh;
+  unsigned short* Sl;
+  const double* scale;  // optional per-p scale (eps^-1), indexed p + p0
+  long p0;
+  int rows, cols, colsp, kvec, pc;  // colsp: padded row length of the K-contiguous output
+  bool by_row;
+  template <typename V>
+  __device__ void operator()(int, int, int m, int n, V v) const {
+    int p = m / rows, r = m - p * rows;
+    int kv = n / cols, j = n - kv * cols;
+    long outer = by_row ? ((long)r * kvec + kv) : ((long)kv * rows + r);
+    long idx = outer * ((long)pc * colsp) + (long)p * colsp + j;
+    double x = (double)v;
+    if (scale) x *= scale[p0 + p];
+    if (S64) {
+      S64[idx] = x;
+    } else {
+      unsigned short h, l;
+      split_bf16(x, h, l);
+      Sh[idx] = h;
+      if (Sl) Sl[idx] = l;
+    }
+  }
+};
+
+// ------------------------------------------------------------------ GEMM dispatch
+template <bool AK, bool BK_, class Epi>
+void gemm64(hipStream_t st, int M, int N, int K, int batch, int ksplit, Opnd<double> A, Opnd<double> B,
+            const double* s, long s_bs, bool lower, Epi epi) {
+  if (M <= 0 || N <= 0 || batch <= 0) return;
+  GemmDims d = make_dims<64, 64>(M, N, std::max(K, 0), batch, ksplit, 16, lower);
+  XG_HIP((launch_gemm_f64<64, 64, AK, BK_>(st, d, A, B, s, s_bs, epi)));
+}
+
+template <bool AK, bool BK_, class Epi>
+void gemm16(hipStream_t st, int M, int N, int K, int batch, int ksplit, Opnd<unsigned short> A,
+            Opnd<unsigned short> B, const float* s, long s_bs, bool split, bool lower, Epi epi) {
+  if (M <= 0 || N <= 0 || batch <= 0) return;
+  GemmDims d = make_dims<128, 128>(M, N, std::max(K, 0), batch, ksplit, 32, lower);
+  XG_HIP((launch_gemm_bf16<128, 128, AK, BK_>(st, d, A, B, s, s_bs, split, epi)));
+}
+
+// Number of K splits so that tiles*splits fills the device several times over.
+int pick_ksplit(long tiles, long K, int bk) {
+  long want = std::max<long>(1, 2048 / std::max<long>(1, tiles));
+  long maxs = std::max<long>(1, K / (4 * bk));
+  return (int)std::max<long>(1, std::min(want, maxs));
+}
+
+void fill_zero(hipStream_t st, double* p, size_t n) {
+  launch_map(st, n, [=] __device__(size_t i) { p[i] = 0.0; });
+}
+
+void split_to_bf16(hipStream_t st, const double* src, unsigned short* hi, unsigned short* lo, size_t n) {
+  launch_map(st, n, [=] __device__(size_t i) {
+    unsigned short h, l;
+    split_bf16(src[i], h, l);
+    hi[i] = h;
+    if (lo) lo[i] = l;
+  });
+}
+
+}  // namespace
+
+// =================================================================== context / memory
+extern "C" const char* xtpgpu_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int xtpgpu_device_count(int* count) {
+  return guarded([&] {
+    XG_REQUIRE(count, XTPGPU_ERR_ARG, "null count");
+    XG_HIP(hipGetDeviceCount(count));
+  });
+}
+
+extern "C" int xtpgpu_ctx_create(int device, unsigned flags, xtpgpu_ctx* out) {
+  return guarded([&] {
+    XG_REQUIRE(out && flags == 0, XTPGPU_ERR_ARG, "xtpgpu_ctx_create: bad arguments");
+    int n = 0;
+    XG_HIP(hipGetDeviceCount(&n));
+    XG_REQUIRE(device >= 0 && device < n, XTPGPU_ERR_ARG, "xtpgpu_ctx_create: device out of range");
+    XG_HIP(hipSetDevice(device));
+    auto c = std::make_unique<xtpgpu_ctx_s>();
+    c->device = device;
+    XG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    *out = c.release();
+  });
+}
+
+extern "C" int xtpgpu_ctx_destroy(xtpgpu_ctx ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    ctx->collect();
+    for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+    delete ctx;
+  });
+}
+
+extern "C" int xtpgpu_ctx_synchronize(xtpgpu_ctx ctx) {
+  return guarded([&] {
+    XG_REQUIRE(ctx, XTPGPU_ERR_ARG, "null ctx");
+    XG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+extern "C" int xtpgpu_ctx_set_profiling(xtpgpu_ctx ctx, int enable) {
+  return guarded([&] {
+    XG_REQUIRE(ctx, XTPGPU_ERR_ARG, "null ctx");
+    ctx->profiling = enable != 0;
+  });
+}
+
+extern "C" int xtpgpu_ctx_timer(xtpgpu_ctx ctx, const char* name, double* total_ms, int64_t* launches, double* flops,
+                                double* bytes) {
+  return guarded([&] {
+    XG_REQUIRE(ctx && name, XTPGPU_ERR_ARG, "null argument");
+    XG_HIP(hipSetDevice(ctx->device));
+    XG_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->collect();
+    KTimer& t = ctx->timer(name);
+    if (total_ms) *total_ms = t.ms;
+    if (launches) *launches = t.launches;
+    if (flops) *flops = t.flops;
+    if (bytes) *bytes = t.bytes;
+  });
+}
+
+extern "C" int xtpgpu_ctx_reset_timers(xtpgpu_ctx ctx) {
+  return guarded([&] {
+    XG_REQUIRE(ctx, XTPGPU_ERR_ARG, "null ctx");
+    XG_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->collect();
+    ctx->timers.clear();
+  });
+}
+
+extern "C" int xtpgpu_malloc(xtpgpu_ctx ctx, int64_t bytes, void** dptr) {
+  return guarded([&] {
+    XG_REQUIRE(ctx && dptr && bytes >= 0, XTPGPU_ERR_ARG, "xtpgpu_malloc: bad arguments");
+    XG_HIP(hipSetDevice(ctx->device));
+    size_t fr = 0, tot = 0;
+    XG_HIP(hipMemGetInfo(&fr, &tot));
+    XG_REQUIRE((size_t)bytes <= fr, XTPGPU_ERR_OOM, "xtpgpu_malloc: request exceeds free device memory");
+    XG_HIP(hipMalloc(dptr, (size_t)std::max<int64_t>(bytes, 1)));
+  });
+}
+
+extern "C" int xtpgpu_free(xtpgpu_ctx ctx, void* dptr) {
+  return guarded([&] {
+    XG_REQUIRE(ctx, XTPGPU_ERR_ARG, "null ctx");
+    XG_HIP(hipStreamSynchronize(ctx->stream));
+    if (dptr) XG_HIP(hipFree(dptr));
+  });
+}
+
+extern "C" int xtpgpu_memcpy_h2d(xtpgpu_ctx ctx, void* dst, const void* src, int64_t bytes) {
+  return guarded([&] {
+    XG_REQUIRE(ctx && bytes >= 0, XTPGPU_ERR_ARG, "bad arguments");
+    XG_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, ctx->stream));
+    XG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+extern "C" int xtpgpu_memcpy_d2h(xtpgpu_ctx ctx, void* dst, const void* src, int64_t bytes) {
+  return guarded([&] {
+    XG_REQUIRE(ctx && bytes >= 0, XTPGPU_ERR_ARG, "bad arguments");
+    XG_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, ctx->stream));
+    XG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+// =================================================================== TCMatrix_gwbse
+extern "C" int xtpgpu_mmn_create(xtpgpu_ctx ctx, int64_t mmin, int64_t mmax, int64_t nmin,
+                                 int64_t nmax, int64_t auxsize, xtpgpu_mmn* out) {
+  return guarded([&] {
+    XG_REQUIRE(ctx && out, XTPGPU_ERR_ARG, "xtpgpu_mmn_create: null argument");
+    XG_REQUIRE(mmax >= mmin && nmax >= nmin && auxsize > 0 && mmin >= 0 && nmin >= 0, XTPGPU_ERR_SHAPE,
+               "xtpgpu_mmn_create: invalid index ranges");
+    XG_HIP(hipSetDevice(ctx->device));
+    auto t = std::make_unique<xtpgpu_mmn_s>();
+    t->ctx = ctx;
+    t->mmin = mmin; t->mmax = mmax; t->nmin = nmin; t->nmax = nmax;
+    t->mtotal = mmax - mmin + 1;
+    t->ntotal = nmax - nmin + 1;
+    t->aux = auxsize;
+    size_t n = (size_t)t->mtotal * t->ntotal * t->aux;
+    t->data.alloc(n * sizeof(double));
+    fill_zero(ctx->stream, t->d(), n);
+    XG_HIP(hipStreamSynchronize(ctx->stream));
+    *out = t.release();
+  });
+}
+
+extern "C" int xtpgpu_mmn_destroy(xtpgpu_mmn mmn) {
+  return guarded([&] {
+    if (!mmn) return;
+    (void)hipSetDevice(mmn->ctx->device);
+    (void)hipStreamSynchronize(mmn->ctx->stream);
+    delete mmn;
+  });
+}
+
+extern "C" int xtpgpu_mmn_sizes(xtpgpu_mmn mmn, int64_t* mtotal, int64_t* ntotal, int64_t* auxsize) {
+  return guarded([&] {
+    XG_REQUIRE(mmn, XTPGPU_ERR_ARG, "null mmn");
+    if (mtotal) *mtotal = mmn->mtotal;
+    if (ntotal) *ntotal = mmn->ntotal;
+    if (auxsize) *auxsize = mmn->aux;
+  });
+}
+
+extern "C" int xtpgpu_mmn_set_slice(xtpgpu_mmn mmn, int64_t m, const double* slice, int64_t ld) {
+  return guarded([&] {
+    XG_REQUIRE(mmn && slice, XTPGPU_ERR_ARG, "null argument");
+    XG_REQUIRE(m >= 0 && m < mmn->mtotal && ld >= mmn->ntotal, XTPGPU_ERR_SHAPE, "xtpgpu_mmn_set_slice: bad slice index / ld");
+    xtpgpu_ctx ctx = mmn->ctx;
+    XG_HIP(hipSetDevice(ctx->device));
+    size_t n = (size_t)ld * mmn->aux;
+    ctx->ws[0].ensure(n * sizeof(double));
+    double* st = ctx->ws[0].as<double>();
+    XG_HIP(hipMemcpyAsync(st, slice, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    double* D = mmn->d();
+    long nt = mmn->ntotal, ps = (long)mmn->pstride();
+    long ldl = ld;
+    launch_map(ctx->stream, (size_t)nt * mmn->aux, [=] __device__(size_t i) {
+      long p = (long)(i / nt), r = (long)(i % nt);
+      D[p * ps + m * nt + r] = st[p * ldl + r];
+    });
+    XG_HIP(hipStreamSynchronize(ctx->stream));
+    mmn->version++;
+  });
+}
+
+extern "C" int xtpgpu_mmn_get_slice(xtpgpu_mmn mmn, int64_t m, double* slice, int64_t ld) {
+  return guarded([&] {
+    XG_REQUIRE(mmn && slice, XTPGPU_ERR_ARG, "null argument");
+    XG_REQUIRE(m >= 0 && m < mmn->mtotal && ld >= mmn->ntotal, XTPGPU_ERR_SHAPE, "xtpgpu_mmn_get_slice: bad slice index / ld");
+    xtpgpu_ctx ctx = mmn->ctx;
+    XG_HIP(hipSetDevice(ctx->device));
+    size_t n = (size_t)ld * mmn->aux;
+    ctx->ws[0].ensure(n * sizeof(double));
+    double* st = ctx->ws[0].as<double>();
+    const double* D = mmn->d();
+    long nt = mmn->ntotal, ps = (long)mmn->pstride();
+    long ldl = ld;
+    launch_map(ctx->stream, n, [=] __device__(size_t i) {
+      long p = (long)(i / ldl), r = (long)(i % ldl);
+      st[i] = r < nt ? D[p * ps + m * nt + r] : 0.0;
+    });
+    XG_HIP(hipMemcpyAsync(slice, st, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    XG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+extern "C" int xtpgpu_mmn_multiply_right(xtpgpu_mmn mmn, const double* A, int64_t lda) {
+  return guarded([&] {
+    XG_REQUIRE(mmn && A, XTPGPU_ERR_ARG, "null argument");
+    XG_REQUIRE(lda >= mmn->aux, XTPGPU_ERR_SHAPE, "xtpgpu_mmn_multiply_right: lda < auxsize");
+    xtpgpu_ctx ctx = mmn->ctx;
+    XG_HIP(hipSetDevice(ctx->device));
+    const long P = mmn->aux;
+    const long MN = (long)mmn->pstride();
+    DevBuf dA((size_t)lda * P * sizeof(double));
+    XG_HIP(hipMemcpyAsync(dA.p, A, (size_t)lda * P * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    DevBuf out((size_t)P * MN * sizeof(double));
+    // new[Q][mn] = sum_P A(P,Q) old[P][mn]  ->  C(m=Q, n=mn), A_op(Q,P) = A[P + Q*lda] (K-contiguous),
+    // B_op(P, mn) = old[P*MN + mn] (N-contiguous).
+    for (long n0 = 0; n0 < MN; n0 += (1L << 30)) {
+      int nn = (int)std::min<long>(MN - n0, 1L << 30);
+      gemm64<true, false>(ctx->stream, (int)P, nn, (int)P, 1, 1, Opnd<double>{dA.as<double>(), nullptr, lda, 0},
+                          Opnd<double>{mmn->d() + n0, nullptr, MN, 0}, nullptr, 0, false,
+                          EpiStore{out.as<double>() + n0, MN, 1, 0, 1.0, 0.0});
+    }
+    XG_HIP(hipStreamSynchronize(ctx->stream));
+    mmn->data = std::move(out);
+    mmn->version++;
+  });
+}
+
+extern "C" int xtpgpu_mmn_fill_ao2mo(xtpgpu_mmn mmn, const double* ao3c, int64_t nbf, const double* C,
+                                     int64_t ldc, int64_t nmo) {
+  return guarded([&] {
+    XG_REQUIRE(mmn && ao3c && C, XTPGPU_ERR_ARG, "null argument");
+    XG_REQUIRE(nbf > 0 && ldc >= nbf && nmo > mmn->mmax && nmo > mmn->nmax, XTPGPU_ERR_SHAPE,
+               "xtpgpu_mmn_fill_ao2mo: MO window outside the coefficient matrix");
+    xtpgpu_ctx ctx = mmn->ctx;
+    XG_HIP(hipSetDevice(ctx->device));
+    const long P = mmn->aux, nb = nbf, mt = mmn->mtotal, nt = mmn->ntotal;
+    DevBuf dT((size_t)P * nb * nb * sizeof(double));
+    DevBuf dC((size_t)ldc * nmo * sizeof(double));
+    XG_HIP(hipMemcpyAsync(dT.p, ao3c, (size_t)P * nb * nb * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    XG_HIP(hipMemcpyAsync(dC.p, C, (size_t)ldc * nmo * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    // step 1: H[Q](m, nu) = sum_mu Cm(mu, m) (Q|mu nu)   (batched over Q)
+    DevBuf dH((size_t)P * mt * nb * sizeof(double));
+    const double* Cm = dC.as<double>() + (long)mmn->mmin * ldc;
+    const double* Cn = dC.as<double>() + (long)mmn->nmin * ldc;
+    // A(m, mu) = Cm[mu + m*ldc] (K-contiguous); B(mu, nu) = T[Q][mu + nu*nb] (K-contiguous)
+    gemm64<true, true>(ctx->stream, (int)mt, (int)nb, (int)nb, (int)P, 1, Opnd<double>{Cm, nullptr, ldc, 0},
+                       Opnd<double>{dT.as<double>(), nullptr, nb, nb * nb}, nullptr, 0, false,
+                       EpiStore{dH.as<double>(), nb, 1, mt * nb, 1.0, 0.0});
+    // step 2: M[Q][m][n] = sum_nu H[Q](m, nu) Cn(nu, n)  ->  data[Q*mt*nt + m*nt + n]
+    gemm64<true, true>(ctx->stream, (int)mt, (int)nt, (int)nb, (int)P, 1,
+                       Opnd<double>{dH.as<double>(), nullptr, nb, mt * nb}, Opnd<double>{Cn, nullptr, ldc, 0},
+                       nullptr, 0, false, EpiStore{mmn->d(), nt, 1, mt * nt, 1.0, 0.0});
+    XG_HIP(hipStreamSynchronize(ctx->stream));
+    mmn->version++;
+  });
+}
+
+extern "C" int xtpgpu_mmn_fill_synthetic(xtpgpu_mmn mmn, uint64_t seed, double sigma0, double decay) {
+  return guarded([&] {
+    XG_REQUIRE(mmn, XTPGPU_ERR_ARG, "null mmn");
+    xtpgpu_ctx ctx = mmn->ctx;
+    XG_HIP(hipSetDevice(ctx->device));
+    const long P = mmn->aux;
+    std::vector<double> sig(P);
+    for (long p = 0; p < P; ++p) sig[p] = sigma0 * std::exp(-decay * (double)p / (double)P);
+    DevBuf ds(P * sizeof(double));
+    XG_HIP(hipMemcpyAsync(ds.p, sig.data(), P * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    const double* s = ds.as<double>();
+    double* D = mmn->d();
+    const long mt = mmn->mtotal, nt = mmn->ntotal, mmin = mmn->mmin, nmin = mmn->nmin;
+    const size_t n = (size_t)P * mt * nt;
+    launch_map(ctx->stream, n, [=] __device__(size_t i) {
+      long p = (long)(i / (size_t)(mt * nt));
+      long r = (long)(i % (size_t)(mt * nt));
+      long m = r / nt + mmin, q = r % nt + nmin;
+      long a = m < q ? m : q, b = m < q ? q : m;
+      D[i] = synth_normal(seed, (uint64_t)p, (uint64_t)a, (uint64_t)b) * s[p];
+    });
+    XG_HIP(hipStreamSynchronize(ctx->stream));
+    mmn->version++;
+  });
+}
+
+// =================================================================== RPA
+namespace {
+
+struct RpaPacked {
+  DevBuf A64;          // [P][K] FP64
+  DevBuf Ah, Al;       // [P][K] bf16 hi / lo
+  long K = 0;
+};
+
+void rpa_pack(xtpgpu_mmn mmn, long n_occ, long n_unocc, bool need64, bool need16, RpaPacked& pk) {
+  hipStream_t st = mmn->ctx->stream;
+  const long P = mmn->aux, K = n_occ * n_unocc, nt = mmn->ntotal, ps = (long)mmn->pstride();
+  const long r0 = nt - n_unocc;
+  pk.K = K;
+  const double* D = mmn->d();
+  if (need64) {
+    pk.A64.alloc((size_t)P * K * sizeof(double));
+    double* A = pk.A64.as<double>();
+    launch_map(st, (size_t)P * K, [=] __device__(size_t i) {
+      long p = (long)(i / K), k = (long)(i % K);
+      long v = k / n_unocc, c = k % n_unocc;
+      A[i] = D[p * ps + v * nt + r0 + c];
+    });
+  }
+  if (need16) {
+    pk.Ah.alloc((size_t)P * K * 2);
+    pk.Al.alloc((size_t)P * K * 2);
+    unsigned short* H = pk.Ah.as<unsigned short>();
+    unsigned short* L = pk.Al.as<unsigned short>();
+    launch_map(st, (size_t)P * K, [=] __device__(size_t i) {
+      long p = (long)(i / K), k = (long)(i % K);
+      long v = k / n_unocc, c = k % n_unocc;
+      unsigned short h, l;
+      split_bf16(D[p * ps + v * nt + r0 + c], h, l);
+      H[i] = h;
+      L[i] = l;
+    });
+  }
+}
+
+void rpa_epsilon_impl(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* energies, int64_t homo, int64_t rpamin,
+                      int64_t rpamax, int kind, const double* omega_re, const double* omega_im, int n_omega,
+                      double eta, int precision, double* out_dev) {
+  XG_REQUIRE(ctx && mmn && energies && omega_re && out_dev, XTPGPU_ERR_ARG, "xtpgpu_rpa_epsilon: null argument");
+  XG_REQUIRE(n_omega > 0, XTPGPU_ERR_ARG, "xtpgpu_rpa_epsilon: n_omega must be positive");
+  XG_REQUIRE(kind >= 0 && kind <= 2, XTPGPU_ERR_ARG, "xtpgpu_rpa_epsilon: unknown kind");
+  XG_REQUIRE(precision == XTPGPU_FP64 || precision == XTPGPU_BF16 || precision == XTPGPU_BF16X3,
+             XTPGPU_ERR_ARG, "xtpgpu_rpa_epsilon: unknown precision");
+  const long lumo = homo + 1;
+  const long n_occ = lumo - rpamin;
+  const long n_unocc = rpamax - lumo + 1;
+  XG_REQUIRE(n_occ > 0 && n_unocc > 0, XTPGPU_ERR_SHAPE, "xtpgpu_rpa_epsilon: empty occupied or virtual range");
+  XG_REQUIRE(n_occ <= mmn->mtotal && n_unocc <= mmn->ntotal, XTPGPU_ERR_SHAPE,
+             "xtpgpu_rpa_epsilon: RPA window exceeds the Mmn tensor");
+  XG_HIP(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const long P = mmn->aux;
+  const long rpatotal = rpamax - rpamin + 1;
+  const long K = n_occ * n_unocc;
+  RpaPacked pk;
+  const bool use16 = precision != XTPGPU_FP64;
+  rpa_pack(mmn, n_occ, n_unocc, !use16, use16, pk);
+
+  // weights d[w][K] (rpa.cc:110-125, 178-190; the complex kind carries the -2 of rpa.cc:197)
+  DevBuf de(rpatotal * sizeof(double)), dw(2 * n_omega * sizeof(double));
+  XG_HIP(hipMemcpyAsync(de.p, energies, rpatotal * sizeof(double), hipMemcpyHostToDevice, st));
+  std::vector<double> w(2 * n_omega, 0.0);
+  for (int i = 0; i < n_omega; ++i) {
+    w[i] = omega_re[i];
+    w[n_omega + i] = omega_im ? omega_im[i] : 0.0;
+  }
+  XG_HIP(hipMemcpyAsync(dw.p, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  DevBuf dd((size_t)n_omega * K * (use16 ? sizeof(float) : sizeof(double)));
+  const double* E = de.as<double>();
+  const double* W = dw.as<double>();
+  double* d64 = use16 ? nullptr : dd.as<double>();
+  float* d32 = use16 ? dd.as<float>() : nullptr;
+  const int nw = n_omega;
+  launch_map(st, (size_t)n_omega * K, [=] __device__(size_t i) {
+    long f = (long)(i / K), k = (long)(i % K);
+    long v = k / n_unocc, c = k % n_unocc;
+    double D = E[n_occ + c] - E[v];
+    double wr = W[f], wi = W[nw + f];
+    double r;
+    if (kind == XTPGPU_EPS_IMAG) {
+      r = 4.0 * D / (D * D + wr * wr);
+    } else if (kind == XTPGPU_EPS_REAL) {
+      double a = D - wr, b = D + wr;
+      r = 2.0 * (a / (a * a + eta * eta) + b / (b * b + eta * eta));
+    } else {
+      double dm = wr - D, dp = wr + D;
+      double s1 = (wi + eta) * (wi + eta), s2 = (wi - eta) * (wi - eta);
+      r = -2.0 * (dm / (dm * dm + s1) - dp / (dp * dp + s2));
+    }
+    if (d64) d64[i] = r; else d32[i] = (float)r;
+  });
+
+  EpiEpsSym epi{out_dev, P};
+  // algorithmic FLOPs: n_omega * K * P * (P+1) (symmetric rank-K update); bytes: M read once + eps written
+  ProfScope prof(ctx, "rpa_eps_gemm", (double)n_omega * K * P * (P + 1),
+                 (double)P * K * (use16 ? (precision == XTPGPU_BF16X3 ? 4 : 2) : 8) + (double)n_omega * P * P * 8);
+  if (!use16) {
+    Opnd<double> A{pk.A64.as<double>(), nullptr, K, 0};
+    gemm64<true, true>(st, (int)P, (int)P, (int)K, n_omega, 1, A, A, d64, K, true, epi);
+  } else {
+    const bool split = precision == XTPGPU_BF16X3;
+    Opnd<unsigned short> A{pk.Ah.as<unsigned short>(), pk.Al.as<unsigned short>(), K, 0};
+    gemm16<true, true>(st, (int)P, (int)P, (int)K, n_omega, 1, A, A, d32, K, split, true, epi);
+  }
+  XG_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace
+
+extern "C" int xtpgpu_rpa_epsilon_device(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* energies, int64_t homo,
+                                         int64_t rpamin, int64_t rpamax, int kind, const double* omega_re,
+                                         const double* omega_im, int n_omega, double eta, int precision,
+                                         double* out_dev) {
+  return guarded([&] {
+    rpa_epsilon_impl(ctx, mmn, energies, homo, rpamin, rpamax, kind, omega_re, omega_im, n_omega, eta, precision,
+                     out_dev);
+  });
+}
+
+extern "C" int xtpgpu_rpa_epsilon(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* energies, int64_t homo,
+                                  int64_t rpamin, int64_t rpamax, int kind, const double* omega_re,
+                                  const double* omega_im, int n_omega, double eta, int precision, double* out) {
+  return guarded([&] {
+    XG_REQUIRE(ctx && mmn && out, XTPGPU_ERR_ARG, "xtpgpu_rpa_epsilon: null argument");
+    const size_t n = (size_t)mmn->aux * mmn->aux * std::max(n_omega, 0);
+    DevBuf o(n * sizeof(double));
+    rpa_epsilon_impl(ctx, mmn, energies, homo, rpamin, rpamax, kind, omega_re, omega_im, n_omega, eta, precision,
+                     o.as<double>());
+    XG_HIP(hipMemcpyAsync(out, o.p, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    XG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+// =================================================================== BSE operator
+// Device layouts (ctp / vtp / Pp = ctotal / vtotal / auxsize rounded up to 8, pads zero):
+//   X   [(kv, v')][ctp]        the k input vectors, BSE index (v', c')
+//   Mcc [P][c1][ctp]           M^P_{c1 c'}            Hd stage 1 (A, K = c')
+//   Bvv [v1][P][vtp]           eps_P M^P_{v1 v'}      Hd stage 2 (B, K = (P, v'))
+//   Mvc [P][v][ctp]            M^P_{v c}              Hx V^T X (A, K = (v,c)); Hd2 stage 1 (A, K = c')
+//   MvcT[(v,c)][Pp]            M^P_{v c}              Hx V W (A, K = P)
+//   McvT[c1][P][vtp]           M^P_{c1 v'}            Hd2 stage 2 (A, K = (P, v'))
+// so that every GEMM reads both operands K-contiguous with 16-byte vectors.
+struct xtpgpu_bseop_s {
+  xtpgpu_ctx ctx = nullptr;
+  int cqp = 0, cx = 0, cd = 0, cd2 = 0;
+  long vt = 0, ct = 0, N = 0, P = 0;
+  long ctp = 0, vtp = 0, Pp = 0;
+  bool hqp_diag = true;
+  unsigned mask = 1;
+  DevBuf hc, hv;        // Hqp blocks (column-major ct x ct / vt x vt)
+  DevBuf hcd, hvd;      // diagonals
+  DevBuf eps;           // eps0^-1 (P)
+  DevBuf diag;          // operator diagonal (N)
+  DevBuf Mcc, Bvv, Mvc, MvcT, McvT;                                  // FP64
+  DevBuf Mcc_h, Mcc_l, Bvv_h, Bvv_l, Mvc_h, Mvc_l, McvT_h, McvT_l;  // BF16 hi / lo
+  // scratch
+  DevBuf xp, xh, xl, s1, s1h, s1l, part, wbuf;
+  size_t s1_bytes = 0;
+};
+
+namespace {
+
+inline long round8(long x) { return (x + 7) & ~7L; }
+
+// grow-only scratch whose unwritten pad positions must read as zero
+void ensure_zeroed(DevBuf& b, size_t bytes, hipStream_t st) {
+  if (bytes <= b.bytes) return;
+  b.alloc(bytes);
+  unsigned char* p = b.as<unsigned char>();
+  launch_map(st, bytes, [=] __device__(size_t i) { p[i] = 0; });
+}
+
+void bse_matmul_impl(xtpgpu_bseop op, int precision, const double* X, long ldx, long k, double* Y, long ldy) {
+  xtpgpu_ctx ctx = op->ctx;
+  hipStream_t st = ctx->stream;
+  const long N = op->N, vt = op->vt, ct = op->ct, P = op->P;
+  const long ctp = op->ctp, vtp = op->vtp, Pp = op->Pp;
+  const long Np = vt * ctp;  // padded length of one input vector
+  XG_REQUIRE(k >= 0 && ldx >= N && ldy >= N, XTPGPU_ERR_SHAPE, "bse matmul: leading dimension smaller than operator size");
+  XG_REQUIRE(precision == XTPGPU_FP64 || ((op->mask >> precision) & 1u), XTPGPU_ERR_ARG,
+             "bse matmul: operator was created without this precision");
+  if (k == 0) return;
+  const bool m16 = precision != XTPGPU_FP64;
+  const bool split = precision == XTPGPU_BF16X3;
+  const bool need16 = m16 && (op->cd || op->cd2);
+
+  // padded copies of X (FP64 always; BF16 hi/lo for the mixed contractions)
+  op->xp.ensure((size_t)Np * k * 8);
+  double* Xp = op->xp.as<double>();
+  unsigned short* Xh = nullptr;
+  unsigned short* Xl = nullptr;
+  if (need16) {
+    op->xh.ensure((size_t)Np * k * 2);
+    Xh = op->xh.as<unsigned short>();
+    if (split) {
+      op->xl.ensure((size_t)Np * k * 2);
+      Xl = op->xl.as<unsigned short>();
+    }
+  }
+  launch_map(st, (size_t)Np * k, [=] __device__(size_t i) {
+    long col = (long)(i / (size_t)ctp), c = (long)(i % (size_t)ctp);  // col = kv*vt + v
+    long kv = col / vt, v = col % vt;
+    double x = c < ct ? X[kv * ldx + v * ct + c] : 0.0;
+    Xp[i] = x;
+    if (Xh) {
+      unsigned short h, l;
+      split_bf16(x, h, l);
+      Xh[i] = h;
+      if (Xl) Xl[i] = l;
+    }
+  });
+  // Y = 0
+  launch_map(st, (size_t)N * k, [=] __device__(size_t i) { Y[(i / N) * ldy + i % N] = 0.0; });
+
+  // ---------------- Hqp (bse_operator.cc:121-132): cqp (H_c^T X - X H_v) per vector
+  if (op->cqp) {
+    ProfScope prof(ctx, "hqp", 2.0 * N * k * (op->hqp_diag ? 1.0 : (double)(ct + vt)), 8.0 * 3.0 * N * k);
+    const double cqp = op->cqp;
+    if (op->hqp_diag) {
+      const double* hc = op->hcd.as<double>();
+      const double* hv = op->hvd.as<double>();
+      launch_map(st, (size_t)N * k, [=] __device__(size_t i) {
+        long kv = (long)(i / N), r = (long)(i % N);
+        long v = r / ct, c = r % ct;
+        Y[kv * ldy + r] += cqp * (hc[c] - hv[v]) * Xp[(kv * vt + v) * ctp + c];
+      });
+    } else {
+      // out_kv(c1, v1) += cqp * sum_c' Hc(c', c1) X_kv(c', v1)   (batched over kv)
+      gemm64<true, true>(st, (int)ct, (int)vt, (int)ct, (int)k, 1, Opnd<double>{op->hc.as<double>(), nullptr, ct, 0},
+                         Opnd<double>{Xp, nullptr, ctp, vt * ctp}, nullptr, 0, false,
+                         EpiStore{Y, 1, ct, ldy, cqp, 1.0});
+      // out_kv(c1, v1) -= cqp * sum_v' X_kv(c1, v') Hv(v', v1)
+      gemm64<false, true>(st, (int)ct, (int)vt, (int)vt, (int)k, 1, Opnd<double>{Xp, nullptr, ctp, vt * ctp},
+                          Opnd<double>{op->hv.as<double>(), nullptr, vt, 0}, nullptr, 0, false,
+                          EpiStore{Y, 1, ct, ldy, -cqp, 1.0});
+    }
+  }
+
+  // aux chunk for the two-stage Hd / Hd2 contraction (first-stage scratch <= ~3 GB)
+  const size_t budget = (size_t)3 << 30;
+  const size_t eb1 = m16 ? 2 * (split ? 2 : 1) : 8;
+  const size_t per_p = (size_t)std::max(ct, vt) * k * vtp * eb1;
+  const long pc = std::max<long>(1, std::min<long>(P, (long)(budget / std::max<size_t>(per_p, 1))));
+
+  // ---------------- Hd (bse_operator.cc:70-89, PrepareMatrix1/2): -cd sum_P eps_P M^P_cc' X M^P_vv'^T
+  if (op->cd) {
+    const double alpha = -(double)op->cd;
+    for (long p0 = 0; p0 < P; p0 += pc) {
+      const long np = std::min(pc, P - p0);
+      const long M1 = np * ct, N1 = k * vt;
+      const double eb = m16 ? (split ? 4.0 : 2.0) : 8.0;
+      const double f1 = 2.0 * M1 * N1 * ct, b1 = eb * ((double)M1 * ct + (double)N * k + (double)M1 * N1);
+      const double f2 = 2.0 * ct * vt * np * vt * k, b2 = eb * ((double)M1 * N1 + (double)vt * np * vt) + 16.0 * N * k;
+      EpiStage1 e1{nullptr, nullptr, nullptr, nullptr, 0, (int)ct, (int)vt, (int)vtp, (int)k, (int)np, true};
+      const size_t s1n = (size_t)ct * k * np * vtp;
+      if (!m16) {
+        ensure_zeroed(op->s1, s1n * 8, st);
+        double* S = op->s1.as<double>();
+        e1.S64 = S;
+        {
+          ProfScope prof(ctx, "hd_stage1", f1, b1);
+          gemm64<true, true>(st, (int)M1, (int)N1, (int)ctp, 1, 1,
+                             Opnd<double>{op->Mcc.as<double>() + p0 * ct * ctp, nullptr, ctp, 0},
+                             Opnd<double>{Xp, nullptr, ctp, 0}, nullptr, 0, false, e1);
+        }
+        ProfScope prof(ctx, "hd_stage2", f2, b2);
+        gemm64<true, true>(st, (int)ct, (int)vt, (int)(np * vtp), (int)k, 1,
+                           Opnd<double>{S, nullptr, k * np * vtp, np * vtp},
+                           Opnd<double>{op->Bvv.as<double>() + p0 * vtp, nullptr, P * vtp, 0}, nullptr, 0, false,
+                           EpiStore{Y, 1, ct, ldy, alpha, 1.0});
+      } else {
+        ensure_zeroed(op->s1h, s1n * 2, st);
+        if (split) ensure_zeroed(op->s1l, s1n * 2, st);
+        unsigned short* Sh = op->s1h.as<unsigned short>();
+        unsigned short* Sl = split ? op->s1l.as<unsigned short>() : nullptr;
+        e1.Sh = Sh;
+        e1.Sl = Sl;
+        {
+          ProfScope prof(ctx, "hd_stage1", f1, b1);
+          gemm16<true, true>(st, (int)M1, (int)N1, (int)ctp, 1, 1,
+                             Opnd<unsigned short>{op->Mcc_h.as<unsigned short>() + p0 * ct * ctp,
+                                                  split ? op->Mcc_l.as<unsigned short>() + p0 * ct * ctp : nullptr,
+                                                  ctp, 0},
+                             Opnd<unsigned short>{Xh, Xl, ctp, 0}, nullptr, 0, split, false, e1);
+        }
+        ProfScope prof(ctx, "hd_stage2", f2, b2);
+        gemm16<true, true>(st, (int)ct, (int)vt, (int)(np * vtp), (int)k, 1,
+                           Opnd<unsigned short>{Sh, Sl, k * np * vtp, np * vtp},
+                           Opnd<unsigned short>{op->Bvv_h.as<unsigned short>() + p0 * vtp,
+                                                split ? op->Bvv_l.as<unsigned short>() + p0 * vtp : nullptr, P * vtp, 0},
+                           nullptr, 0, split, false, EpiStore{Y, 1, ct, ldy, alpha, 1.0});
+      }
+    }
+  }
+
+  // ---------------- Hd2 (bse_operator.cc:74-86, Hd2 branch): -cd2 sum_P eps_P M^P_{c1 v'} M^P_{v1 c'} X_{v'c'}
+  if (op->cd2) {
+    ProfScope prof(ctx, "hd2", 4.0 * P * vt * vt * ct * k, (m16 ? (split ? 4.0 : 2.0) : 8.0) * 2.0 * P * N + 16.0 * N * k);
+    const double alpha = -(double)op->cd2;
+    for (long p0 = 0; p0 < P; p0 += pc) {
+      const long np = std::min(pc, P - p0);
+      const long M1 = np * vt, N1 = k * vt;
+      EpiStage1 e1{nullptr, nullptr, nullptr, op->eps.as<double>(), p0, (int)vt, (int)vt, (int)vtp, (int)k, (int)np,
+                   false};
+      const size_t s1n = (size_t)vt * k * np * vtp;
+      if (!m16) {
+        ensure_zeroed(op->s1, s1n * 8, st);
+        double* S = op->s1.as<double>();
+        e1.S64 = S;
+        gemm64<true, true>(st, (int)M1, (int)N1, (int)ctp, 1, 1,
+                           Opnd<double>{op->Mvc.as<double>() + p0 * Np, nullptr, ctp, 0},
+                           Opnd<double>{Xp, nullptr, ctp, 0}, nullptr, 0, false, e1);
+        gemm64<true, true>(st, (int)ct, (int)vt, (int)(np * vtp), (int)k, 1,
+                           Opnd<double>{op->McvT.as<double>() + p0 * vtp, nullptr, P * vtp, 0},
+                           Opnd<double>{S, nullptr, np * vtp, vt * np * vtp}, nullptr, 0, false,
+                           EpiStore{Y, 1, ct, ldy, alpha, 1.0});
+      } else {
+        ensure_zeroed(op->s1h, s1n * 2, st);
+        if (split) ensure_zeroed(op->s1l, s1n * 2, st);
+        unsigned short* Sh = op->s1h.as<unsigned short>();
+        unsigned short* Sl = split ? op->s1l.as<unsigned short>() : nullptr;
+        e1.Sh = Sh;
+        e1.Sl = Sl;
+        gemm16<true, true>(st, (int)M1, (int)N1, (int)ctp, 1, 1,
+                           Opnd<unsigned short>{op->Mvc_h.as<unsigned short>() + p0 * Np,
+                                                split ? op->Mvc_l.as<unsigned short>() + p0 * Np : nullptr, ctp, 0},
+                           Opnd<unsigned short>{Xh, Xl, ctp, 0}, nullptr, 0, split, false, e1);
+        gemm16<true, true>(st, (int)ct, (int)vt, (int)(np * vtp), (int)k, 1,
+                           Opnd<unsigned short>{op->McvT_h.as<unsigned short>() + p0 * vtp,
+                                                split ? op->McvT_l.as<unsigned short>() + p0 * vtp : nullptr, P * vtp, 0},
+                           Opnd<unsigned short>{Sh, Sl, np * vtp, vt * np * vtp}, nullptr, 0, split, false,
+                           EpiStore{Y, 1, ct, ldy, alpha, 1.0});
+      }
+    }
+  }
+
+  // ---------------- Hx (bse_operator.cc:98-116, MultiplyBlocks): cx V (V^T X), always FP64
+  if (op->cx) {
+    ProfScope prof(ctx, "hx", 4.0 * N * P * k, 8.0 * (2.0 * N * P + 2.0 * N * k));
+    const int tiles = (int)(((P + 63) / 64) * ((k + 63) / 64));
+    const int ks = pick_ksplit(tiles, Np, 16);
+    GemmDims dd = make_dims<64, 64>((int)P, (int)k, (int)Np, 1, ks, 16, false);
+    op->part.ensure((size_t)dd.ksplit * P * k * sizeof(double));
+    ensure_zeroed(op->wbuf, (size_t)Pp * k * sizeof(double), st);
+    double* part = op->part.as<double>();
+    double* Wb = op->wbuf.as<double>();
+    gemm64<true, true>(st, (int)P, (int)k, (int)Np, 1, ks, Opnd<double>{op->Mvc.as<double>(), nullptr, Np, 0},
+                       Opnd<double>{Xp, nullptr, Np, 0}, nullptr, 0, false, EpiPartial{part, (int)P, (int)k, 1});
+    // W[kv][Pp] (pads stay zero)
+    const int nsp = dd.ksplit;
+    launch_map(st, (size_t)P * k, [=] __device__(size_t i) {
+      long kv = (long)(i / P), p = (long)(i % P);
+      double acc = 0.0;
+      for (int q = 0; q < nsp; ++q) acc += part[(long)q * P * k + i];
+      Wb[kv * Pp + p] = acc;
+    });
+    gemm64<true, true>(st, (int)N, (int)k, (int)Pp, 1, 1, Opnd<double>{op->MvcT.as<double>(), nullptr, Pp, 0},
+                       Opnd<double>{Wb, nullptr, Pp, 0}, nullptr, 0, false,
+                       EpiStore{Y, 1, ldy, 0, (double)op->cx, 1.0});
+  }
+}
+
+}  // namespace
+
+extern "C" int xtpgpu_bse_create(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* eps0inv, const double* Hqp,
+                                 int64_t ldhqp, int64_t homo, int64_t rpamin, int64_t qpmin, int64_t vmin,
+                                 int64_t cmax, int cqp, int cx, int cd, int cd2, unsigned precision_mask,
+                                 xtpgpu_bseop* out) {
+  return guarded([&] {
+    (void)qpmin;
+    XG_REQUIRE(ctx && mmn && out, XTPGPU_ERR_ARG, "xtpgpu_bse_create: null argument");
+    XG_REQUIRE(!(cd != 0 && cd2 != 0), XTPGPU_ERR_ARG, "Hamiltonian cannot contain Hd and Hd2 at the same time");
+    XG_REQUIRE(cqp == 0 || Hqp, XTPGPU_ERR_ARG, "xtpgpu_bse_create: Hqp required when cqp != 0");
+    XG_REQUIRE((cd == 0 && cd2 == 0) || eps0inv, XTPGPU_ERR_ARG, "xtpgpu_bse_create: eps0inv required for Hd/Hd2");
+    auto op = std::make_unique<xtpgpu_bseop_s>();
+    op->ctx = ctx;
+    op->cqp = cqp; op->cx = cx; op->cd = cd; op->cd2 = cd2;
+    op->mask = (precision_mask | 1u) & ((1u << XTPGPU_FP64) | (1u << XTPGPU_BF16) | (1u << XTPGPU_BF16X3));
+    // BSE_OPERATOR::configure (bse_operator.cc:28-37)
+    const long bse_vmax = homo, bse_cmin = homo + 1;
+    const long vt = bse_vmax - vmin + 1, ct = cmax - bse_cmin + 1;
+    XG_REQUIRE(vt > 0 && ct > 0, XTPGPU_ERR_SHAPE, "xtpgpu_bse_create: empty BSE window");
+    // slice index / row index = level - rpamin (TCMatrix_gwbse for the BSE starts at rpamin)
+    const long vmin_r = vmin - rpamin, cmin_r = bse_cmin - rpamin;
+    XG_REQUIRE(vmin_r >= 0 && cmin_r + ct <= mmn->mtotal && cmin_r + ct <= mmn->ntotal, XTPGPU_ERR_SHAPE,
+               "xtpgpu_bse_create: BSE window exceeds the Mmn tensor");
+    op->vt = vt; op->ct = ct; op->N = vt * ct; op->P = mmn->aux;
+    op->ctp = round8(ct); op->vtp = round8(vt); op->Pp = round8(op->P);
+    const long P = op->P, N = op->N, nt = mmn->ntotal, ps = (long)mmn->pstride();
+    const long ctp = op->ctp, vtp = op->vtp, Pp = op->Pp, Np = vt * ctp;
+    XG_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const double* D = mmn->d();
+
+    // Hqp blocks (BSE-sized, bse.cc:147-186)
+    const long hs = vt + ct;
+    if (cqp) {
+      XG_REQUIRE(ldhqp >= hs, XTPGPU_ERR_SHAPE, "xtpgpu_bse_create: ldhqp < vtotal+ctotal");
+      std::vector<double> hc(ct * ct), hv(vt * vt), hcd(ct), hvd(vt);
+      bool diag = true;
+      for (long j = 0; j < hs; ++j)
+        for (long i = 0; i < hs; ++i)
+          if (i != j && Hqp[j * ldhqp + i] != 0.0) diag = false;
+      for (long j = 0; j < ct; ++j)
+        for (long i = 0; i < ct; ++i) hc[j * ct + i] = Hqp[(vt + j) * ldhqp + vt + i];
+      for (long j = 0; j < vt; ++j)
+        for (long i = 0; i < vt; ++i) hv[j * vt + i] = Hqp[j * ldhqp + i];
+      for (long i = 0; i < ct; ++i) hcd[i] = Hqp[(vt + i) * ldhqp + vt + i];
+      for (long i = 0; i < vt; ++i) hvd[i] = Hqp[i * ldhqp + i];
+      op->hqp_diag = diag;
+      op->hc.alloc(hc.size() * 8); op->hv.alloc(hv.size() * 8);
+      op->hcd.alloc(ct * 8); op->hvd.alloc(vt * 8);
+      XG_HIP(hipMemcpyAsync(op->hc.p, hc.data(), hc.size() * 8, hipMemcpyHostToDevice, st));
+      XG_HIP(hipMemcpyAsync(op->hv.p, hv.data(), hv.size() * 8, hipMemcpyHostToDevice, st));
+      XG_HIP(hipMemcpyAsync(op->hcd.p, hcd.data(), ct * 8, hipMemcpyHostToDevice, st));
+      XG_HIP(hipMemcpyAsync(op->hvd.p, hvd.data(), vt * 8, hipMemcpyHostToDevice, st));
+      XG_HIP(hipStreamSynchronize(st));
+    }
+    op->eps.alloc(P * 8);
+    if (eps0inv) {
+      XG_HIP(hipMemcpyAsync(op->eps.p, eps0inv, P * 8, hipMemcpyHostToDevice, st));
+    } else {
+      fill_zero(st, op->eps.as<double>(), P);
+    }
+    const double* E = op->eps.as<double>();
+    const bool need16 = ((op->mask >> XTPGPU_BF16) & 1u) || ((op->mask >> XTPGPU_BF16X3) & 1u);
+
+    auto el = [=] __device__(long p, long slice, long row) { return D[p * ps + slice * nt + row]; };
+    if (cd) {
+      op->Mcc.alloc((size_t)P * ct * ctp * 8);
+      double* A = op->Mcc.as<double>();
+      launch_map(st, (size_t)P * ct * ctp, [=] __device__(size_t i) {
+        long p = (long)(i / (size_t)(ct * ctp)), r = (long)(i % (size_t)(ct * ctp));
+        long c1 = r / ctp, c2 = r % ctp;
+        A[i] = c2 < ct ? el(p, cmin_r + c1, cmin_r + c2) : 0.0;
+      });
+      op->Bvv.alloc((size_t)vt * P * vtp * 8);
+      double* B = op->Bvv.as<double>();
+      launch_map(st, (size_t)vt * P * vtp, [=] __device__(size_t i) {
+        long v1 = (long)(i / (size_t)(P * vtp)), r = (long)(i % (size_t)(P * vtp));
+        long p = r / vtp, v2 = r % vtp;
+        B[i] = v2 < vt ? E[p] * el(p, vmin_r + v1, vmin_r + v2) : 0.0;
+      });
+      if (need16) {
+        op->Mcc_h.alloc((size_t)P * ct * ctp * 2); op->Mcc_l.alloc((size_t)P * ct * ctp * 2);
+        split_to_bf16(st, A, op->Mcc_h.as<unsigned short>(), op->Mcc_l.as<unsigned short>(), (size_t)P * ct * ctp);
+        op->Bvv_h.alloc((size_t)vt * P * vtp * 2); op->Bvv_l.alloc((size_t)vt * P * vtp * 2);
+        split_to_bf16(st, B, op->Bvv_h.as<unsigned short>(), op->Bvv_l.as<unsigned short>(), (size_t)vt * P * vtp);
+      }
+    }
+    if (cx || cd2) {
+      op->Mvc.alloc((size_t)P * Np * 8);
+      double* A = op->Mvc.as<double>();
+      launch_map(st, (size_t)P * Np, [=] __device__(size_t i) {
+        long p = (long)(i / (size_t)Np), r = (long)(i % (size_t)Np);
+        long v = r / ctp, c = r % ctp;
+        A[i] = c < ct ? el(p, vmin_r + v, cmin_r + c) : 0.0;
+      });
+      if (need16 && cd2) {
+        op->Mvc_h.alloc((size_t)P * Np * 2); op->Mvc_l.alloc((size_t)P * Np * 2);
+        split_to_bf16(st, A, op->Mvc_h.as<unsigned short>(), op->Mvc_l.as<unsigned short>(), (size_t)P * Np);
+      }
+    }
+    if (cx) {
+      op->MvcT.alloc((size_t)N * Pp * 8);
+      double* A = op->MvcT.as<double>();
+      launch_map(st, (size_t)N * Pp, [=] __device__(size_t i) {
+        long r = (long)(i / (size_t)Pp), p = (long)(i % (size_t)Pp);
+        long v = r / ct, c = r % ct;
+        A[i] = p < P ? el(p, vmin_r + v, cmin_r + c) : 0.0;
+      });
+    }
+    if (cd2) {
+      op->McvT.alloc((size_t)ct * P * vtp * 8);
+      double* A = op->McvT.as<double>();
+      launch_map(st, (size_t)ct * P * vtp, [=] __device__(size_t i) {
+        long c1 = (long)(i / (size_t)(P * vtp)), r = (long)(i % (size_t)(P * vtp));
+        long p = r / vtp, v2 = r % vtp;
+        A[i] = v2 < vt ? el(p, cmin_r + c1, vmin_r + v2) : 0.0;
+      });
+      if (need16) {
+        op->McvT_h.alloc((size_t)ct * P * vtp * 2); op->McvT_l.alloc((size_t)ct * P * vtp * 2);
+        split_to_bf16(st, A, op->McvT_h.as<unsigned short>(), op->McvT_l.as<unsigned short>(), (size_t)ct * P * vtp);
+      }
+    }
+    // BSE_OPERATOR::diagonal (bse_operator.cc:134-175)
+    op->diag.alloc((size_t)N * 8);
+    {
+      double* dg = op->diag.as<double>();
+      const double* hcd = cqp ? op->hcd.as<double>() : nullptr;
+      const double* hvd = cqp ? op->hvd.as<double>() : nullptr;
+      const double fx = cx, fqp = cqp, fd = cd, fd2 = cd2;
+      launch_map(st, (size_t)N, [=] __device__(size_t i) {
+        long v = (long)i / ct, c = (long)i % ct;
+        double e = 0.0;
+        if (fx != 0.0) {
+          double s = 0.0;
+          for (long p = 0; p < P; ++p) { double m = el(p, vmin_r + v, cmin_r + c); s += m * m; }
+          e += fx * s;
+        }
+        if (fqp != 0.0) e += fqp * (hcd[c] - hvd[v]);
+        if (fd != 0.0) {
+          double s = 0.0;
+          for (long p = 0; p < P; ++p) s += el(p, cmin_r + c, cmin_r + c) * E[p] * el(p, vmin_r + v, vmin_r + v);
+          e -= fd * s;
+        }
+        if (fd2 != 0.0) {
+          double s = 0.0;
+          for (long p = 0; p < P; ++p) s += el(p, cmin_r + c, vmin_r + v) * E[p] * el(p, vmin_r + v, cmin_r + c);
+          e -= fd2 * s;
+        }
+        dg[i] = e;
+      });
+    }
+    XG_HIP(hipStreamSynchronize(st));
+    *out = op.release();
+  });
+}
+
+extern "C" int xtpgpu_bse_destroy(xtpgpu_bseop op) {
+  return guarded([&] {
+    if (!op) return;
+    (void)hipSetDevice(op->ctx->device);
+    (void)hipStreamSynchronize(op->ctx->stream);
+    delete op;
+  });
+}
+
+extern "C" int xtpgpu_bse_size(xtpgpu_bseop op, int64_t* size) {
+  return guarded([&] {
+    XG_REQUIRE(op && size, XTPGPU_ERR_ARG, "null argument");
+    *size = op->N;
+  });
+}
+
+extern "C" int xtpgpu_bse_diagonal(xtpgpu_bseop op, double* out) {
+  return guarded([&] {
+    XG_REQUIRE(op && out, XTPGPU_ERR_ARG, "null argument");
+    XG_HIP(hipSetDevice(op->ctx->device));
+    XG_HIP(hipMemcpyAsync(out, op->diag.p, op->N * 8, hipMemcpyDeviceToHost, op->ctx->stream));
+    XG_HIP(hipStreamSynchronize(op->ctx->stream));
+  });
+}
+
+extern "C" int xtpgpu_bse_matmul_device(xtpgpu_bseop op, int precision, const double* X, int64_t ldx, int64_t k,
+                                        double* Y, int64_t ldy) {
+  return guarded([&] {
+    XG_REQUIRE(op && X && Y, XTPGPU_ERR_ARG, "null argument");
+    XG_HIP(hipSetDevice(op->ctx->device));
+    bse_matmul_impl(op, precision, X, ldx, k, Y, ldy);
+    XG_HIP(hipStreamSynchronize(op->ctx->stream));
+  });
+}
+
+extern "C" int xtpgpu_bse_matmul(xtpgpu_bseop op, int precision, const double* X, int64_t ldx, int64_t k, double* Y,
+                                 int64_t ldy) {
+  return guarded([&] {
+    XG_REQUIRE(op && X && Y, XTPGPU_ERR_ARG, "null argument");
+    XG_REQUIRE(ldx >= op->N && ldy >= op->N && k >= 0, XTPGPU_ERR_SHAPE, "xtpgpu_bse_matmul: bad leading dimension");
+    XG_HIP(hipSetDevice(op->ctx->device));
+    hipStream_t st = op->ctx->stream;
+    const size_t n = (size_t)op->N * k;
+    DevBuf dx(n * 8), dy(n * 8);
+    XG_HIP(hipMemcpy2DAsync(dx.p, op->N * 8, X, ldx * 8, op->N * 8, k, hipMemcpyHostToDevice, st));
+    bse_matmul_impl(op, precision, dx.as<double>(), op->N, k, dy.as<double>(), op->N);
+    XG_HIP(hipMemcpy2DAsync(Y, ldy * 8, dy.p, op->N * 8, op->N * 8, k, hipMemcpyDeviceToHost, st));
+    XG_HIP(hipStreamSynchronize(st));
+  });
+}
+
+#include "davidson.inc"
