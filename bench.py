@@ -99,6 +99,10 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-7)
     ap.add_argument("--mixed", default="bf16x3", choices=["bf16", "bf16x3"])
     ap.add_argument("--mixed-tol", type=float, default=1e-4)
+    ap.add_argument("--sigma0", type=float, default=0.01,
+                    help="amplitude of sigma_P = sigma0 exp(-3P/n_aux); 0.3 is the literal config value, "
+                         "on which the reference Davidson does not converge (exchange ~100x physical)")
+    ap.add_argument("--iter-max", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skip-fp64-solve", action="store_true")
     args = ap.parse_args()
@@ -114,7 +118,7 @@ def main():
     ctx = px.Context(0)
     log(f"[bench] o={o} v={v} P={P}: filling synthetic Mmn ({n*n*P*8/1e9:.1f} GB FP64)")
     Mmn = px.TCMatrix_gwbse(ctx).Initialize(P, 0, n - 1, 0, n - 1)
-    Mmn.fill_synthetic(args.seed, 0.3, 3.0)
+    Mmn.fill_synthetic(args.seed, args.sigma0, 3.0)
     e = synthetic_energies(args.seed, o, v)
     rpa = px.RPA(Mmn)
     rpa.configure(o - 1, 0, n - 1)
@@ -171,7 +175,7 @@ def main():
         ds.set_correction("DPR")
         ds.set_size_update("safe")
         ds.set_max_search_space(10 * args.roots)
-        ds.set_iter_max(100)
+        ds.set_iter_max(args.iter_max)
         if prec != "fp64":
             ds.set_precision(prec, refine_tolerance=args.tol, refine_max=6)
         ctx.reset_timers()
@@ -230,6 +234,7 @@ def main():
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3/fp32+fp64",
         "data": "synthetic",
         "config": {"workload": "synthetic single segment (configs[1])", "n_occ": o, "n_virt": v, "n_aux": P,
+                   "sigma0": args.sigma0,
                    "n_freq": 12, "roots": args.roots, "tol": args.tol, "mixed": args.mixed},
         "bse_tda_solve_s": {"mixed": t_mixed, "fp64": res.get("fp64", {}).get("wall_s"),
                             "speedup": (res["fp64"]["wall_s"] / t_mixed) if "fp64" in res else None},

@@ -133,6 +133,57 @@ int xtpgpu_bse_matmul(xtpgpu_bseop op, int precision, const double* X, int64_t l
 int xtpgpu_bse_matmul_device(xtpgpu_bseop op, int precision, const double* X, int64_t ldx,
                              int64_t k, double* Y, int64_t ldy);
 
+/* ---------------------------------------------------------------- screening, GW */
+typedef struct xtpgpu_sigma_s* xtpgpu_sigma;
+
+/* BSE::SetupDirectInteractionOperator (bse.cc:188-204): eps_r(energy) on the device, its
+ * eigendecomposition (rocSOLVER), Mmn <- Mmn * U in place, eps0inv[P] = 1/lambda_P (0 below 1e-8). */
+int xtpgpu_bse_setup_screening(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* energies, int64_t homo,
+                               int64_t rpamin, int64_t rpamax, double energy, double* eps0inv);
+/* Sigma_base::CalcExchangeMatrix (sigma_base.cc:36-52): out (qptotal x qptotal, column-major)
+ * = -sum_{n<occ,P} M^P_{l1 n} M^P_{l2 n}. */
+int xtpgpu_sigma_x(xtpgpu_ctx ctx, xtpgpu_mmn mmn, int64_t homo, int64_t rpamin, int64_t qpmin,
+                   int64_t qpmax, double* out);
+/* PPM::PPM_construct_parameters (ppm.cc:31-59) + Sigma_PPM::PrepareScreening (sigma_ppm.cc:32-35):
+ * eps_r(0), eps_i(0.5) on the device, phi = eigenvectors of eps_r(0), weight = 1 - 1/lambda,
+ * freq from diag((phi^T eps_i phi)^-1); then Mmn <- Mmn * phi in place. */
+int xtpgpu_ppm_prepare(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* energies, int64_t homo,
+                       int64_t rpamin, int64_t rpamax, double* ppm_weight, double* ppm_freq);
+/* Sigma_PPM evaluator bound to a (PPM-rotated) tensor (sigma_ppm.h; options sigma_base.h:47-57). */
+int xtpgpu_sigma_ppm_create(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* rpa_energies,
+                            const double* ppm_weight, const double* ppm_freq, int64_t homo,
+                            int64_t rpamin, int64_t qpmin, double eta, xtpgpu_sigma* out);
+int xtpgpu_sigma_destroy(xtpgpu_sigma s);
+/* Sigma_PPM::CalcCorrelationDiagElement / ...Derivative (sigma_ppm.cc:37-89), batched over n
+ * (gw_level, omega) pairs; sigma / dsigma may be null. */
+int xtpgpu_sigma_ppm_diag(xtpgpu_sigma s, int64_t n, const int64_t* gw_levels, const double* omegas,
+                          double* sigma, double* dsigma);
+/* Sigma_base::CalcCorrelationOffDiag with Sigma_PPM::CalcCorrelationOffDiagElement
+ * (sigma_base.cc:65-78, sigma_ppm.cc:91-126); the diagonal of out is left zero. */
+int xtpgpu_sigma_ppm_offdiag(xtpgpu_sigma s, int64_t qptotal, const double* frequencies, double* out);
+
+typedef struct {
+  int solver;                    /* 0 = fixedpoint (then grid, then linearisation); 1 = grid (then linearisation) */
+  double qp_solver_alpha;        /* Newton damping, gw.h:70 (0.75) */
+  double g_sc_limit;             /* QP tolerance (1e-5) */
+  int g_sc_max_iterations;       /* Newton iterations (50) */
+  double full_window_half_width; /* grid half width, Ha (0.75; legacy steps*spacing/2) */
+  double dense_spacing;          /* grid spacing, Ha (0.002) */
+  int restrict_search;           /* occupied roots < 0, virtual roots > virtual_min_energy */
+  double zero_margin;            /* 1e-6 */
+  double virtual_min_energy;     /* -0.1 */
+  double min_accepted_Z;         /* 0.05 */
+  double max_accepted_Z;         /* 1.5 */
+  int bisection_max_iter;        /* 200 */
+} xtpgpu_qp_opts;
+
+/* GW::SolveQP (gw.cc:323-410): solve w = intercept_l + Sigma_c(l, w) for every QP level,
+ * evaluating Sigma_c for all unsettled levels in one device pass per step. out[qptotal] = QP
+ * energies; converged[l] = 1 where the fixed point or grid search succeeded. */
+int xtpgpu_gw_solve_qp(xtpgpu_sigma s, int64_t qptotal, const double* intercepts,
+                       const double* frequencies, const xtpgpu_qp_opts* opts, double* out,
+                       int* converged);
+
 /* ---------------------------------------------------------------- Davidson */
 typedef struct {
   double tolerance;        /* residual-norm tolerance (DavidsonSolver::set_tolerance, davidsonsolver.cc:122-134) */

@@ -301,6 +301,7 @@ class DavidsonOracle:
         AV = None
         T = None
         self.info_converged = False
+        self.history = []
         for it in range(self.iter_max):
             self.num_iterations = it
             if it == 0:
@@ -323,6 +324,7 @@ class DavidsonOracle:
             q = V @ U
             res = AV @ U - q * lam[None, :]
             rn = np.linalg.norm(res, axis=0)
+            self.history.append(float(rn[:neigen].max()))
             conv = rn[:size_update] < self.tol
             if np.all(conv[:neigen]):
                 self.info_converged = True
@@ -416,3 +418,130 @@ def synthetic_energies(seed: int, n_occ: int, n_virt: int):
     eo = np.sort(rng.uniform(-20.0, -5.0, n_occ))
     ev = np.sort(rng.uniform(0.0, 30.0, n_virt))
     return np.concatenate([eo, ev]) / HRT2EV
+
+
+# ---------------------------------------------------------------- GW: exchange, PPM, Sigma_c
+def sigma_x(M: np.ndarray, homo: int, rpamin: int, qpmin: int, qpmax: int) -> np.ndarray:
+    """Sigma_base::CalcExchangeMatrix -- xtp/src/libxtp/gwbse/sigma_base.cc:36-52:
+    Sx[l1,l2] = -sum_{n<occ} sum_P M^P_{l1 n} M^P_{l2 n}."""
+    occ = homo - rpamin + 1
+    q0 = qpmin - rpamin
+    A = M[q0:q0 + qpmax - qpmin + 1, :occ, :].reshape(qpmax - qpmin + 1, -1)
+    return -A @ A.T
+
+
+def ppm_construct(M: np.ndarray, energies: np.ndarray, homo: int, rpamin: int, rpamax: int,
+                  screening_r: float = 0.0, screening_i: float = 0.5):
+    """PPM::PPM_construct_parameters -- xtp/src/libxtp/gwbse/ppm.cc:31-59 (defaults ppm.h:34).
+    Returns (phi, weight, freq)."""
+    er = rpa_epsilon(M, energies, homo, rpamin, rpamax, screening_r, "real")
+    lam, phi = np.linalg.eigh(er)
+    weight = 1.0 - 1.0 / lam
+    ei = rpa_epsilon(M, energies, homo, rpamin, rpamax, screening_i, "imag")
+    inv = np.linalg.inv(phi.T @ ei @ phi)
+    freq = np.empty_like(weight)
+    for i in range(len(weight)):
+        if weight[i] < 1e-5:
+            weight[i] = 0.0
+            freq[i] = 0.5
+        else:
+            nom = inv[i, i] - 1.0
+            frac = -1.0 * nom / (nom + weight[i]) * screening_i * screening_i
+            freq[i] = np.sqrt(abs(frac))
+    return phi, weight, freq
+
+
+def sigma_ppm_diag(M, energies, weight, freq, homo, rpamin, qpmin, eta, level, omega):
+    """Sigma_PPM::CalcCorrelationDiagElement and ...Derivative -- sigma_ppm.cc:37-89.
+    M is the PPM-rotated tensor (Mmn * phi, sigma_ppm.cc:32-35). Returns (sigma, dsigma/domega)."""
+    lumo = homo + 1
+    eta2 = eta * eta
+    m = M[level + qpmin - rpamin]            # (nsize x P)
+    keep = weight >= 1e-9
+    W, F = weight[keep], freq[keep]
+    M2 = m[:, keep] ** 2                     # (n x P')
+    t = omega - energies[:, None] + np.where(np.arange(m.shape[0])[:, None] < lumo, F[None, :], -F[None, :])
+    den = t * t + eta2
+    fac = 0.5 * W * F
+    s = float(np.sum(fac[None, :] * M2 * t / den))
+    ds = float(np.sum(fac[None, :] * (eta2 - t * t) * M2 / (den * den)))
+    return s, ds
+
+
+def sigma_ppm_offdiag(M, energies, weight, freq, homo, rpamin, qpmin, eta, l1, l2, w1, w2):
+    """Sigma_PPM::CalcCorrelationOffDiagElement -- sigma_ppm.cc:91-126."""
+    lumo = homo + 1
+    eta2 = eta * eta
+    a = M[l1 + qpmin - rpamin]
+    b = M[l2 + qpmin - rpamin]
+    keep = weight >= 1e-9
+    W, F = weight[keep], freq[keep]
+    ab = a[:, keep] * b[:, keep]
+    occ = np.arange(a.shape[0])[:, None] < lumo
+    temp = energies[:, None] + np.where(occ, -F[None, :], F[None, :])
+    t1 = w1 - temp
+    t2 = w2 - temp
+    fac = 0.25 * W * F
+    return float(np.sum(fac[None, :] * (t1 / (t1 * t1 + eta2) + t2 / (t2 * t2 + eta2)) * ab))
+
+
+def qp_fixed_point(sigma_fn, intercept, omega0, alpha=0.75, tol=1e-5, max_iter=50):
+    """GW::SolveQP_FixedPoint (gw.cc:741-757) with NewtonRapson (newton_rapson.h:49-70) on
+    f(w) = Sigma_c(w) + intercept - w. Returns the root or None."""
+    x = omega0
+    for _ in range(max_iter):
+        s, ds = sigma_fn(x)
+        f = s + intercept - x
+        df = ds - 1.0
+        if abs(df) < 1e-12:
+            return None
+        step = -alpha * f / df
+        if abs(step) < tol:
+            return x
+        x += step
+    return None
+
+
+def qp_grid(sigma_fn, intercept, omega0, occupied, half_width=0.75, spacing=0.002, restrict=True,
+            zero_margin=1e-6, virtual_min=-0.1, tol=1e-5, min_Z=0.05, max_Z=1.5, bis_max=200):
+    """Dense grid QP search (GW::SolveQP_Grid, gw.cc:677-739; candidate acceptance and score
+    qp_solver_utils.h:330-402): sign changes of f(w) = Sigma_c(w) + intercept - w on
+    [w0-hw, w0+hw] (restricted to w < 0 for occupied / w > virtual_min for virtual levels),
+    bisection to tol, Z = 1/(1 - dSigma/dw), accepted for min_Z <= Z <= max_Z, best score
+    Z - 0.1|w - w0|. Returns the root or None."""
+    left, right = omega0 - half_width, omega0 + half_width
+    if restrict:
+        if occupied:
+            right = min(right, -zero_margin)
+        else:
+            left = max(left, virtual_min)
+    npts = int(np.floor(2.0 * half_width / spacing + 0.5)) + 1
+    xs = [omega0 - half_width + spacing * k for k in range(npts)]
+    xs = [x for x in xs if left - 1e-14 <= x <= right + 1e-14]
+    fs = [sigma_fn(x)[0] + intercept - x for x in xs]
+    best, best_score = None, -np.inf
+    for k in range(len(xs) - 1):
+        f0, f1 = fs[k], fs[k + 1]
+        if not ((f0 < 0) != (f1 < 0) or f0 == 0.0):
+            continue
+        a, b, fa = xs[k], xs[k + 1], f0
+        for _ in range(bis_max):
+            if b - a <= tol:
+                break
+            m = 0.5 * (a + b)
+            fm = sigma_fn(m)[0] + intercept - m
+            if (fm < 0) == (fa < 0):
+                a, fa = m, fm
+            else:
+                b = m
+        w = 0.5 * (a + b)
+        s, ds = sigma_fn(w)
+        deriv = ds - 1.0
+        Z = -1.0 / deriv if deriv != 0 else np.inf
+        res = s + intercept - w
+        if not np.isfinite(Z) or Z <= 0 or Z < min_Z or Z > max_Z or abs(res) > 10 * tol:
+            continue
+        score = Z - 0.1 * abs(w - omega0)
+        if score > best_score:
+            best, best_score = w, score
+    return best

@@ -121,10 +121,10 @@ def test_bse_operator_fixtures_gpu(ctx, ints, name, c):  # test_bse_operator.cc:
     assert np.abs(H - ref).max() < 1e-12
 
 
-def random_bse_case(seed, vt=7, ct=33, P=70, rpamin=0, vmin_off=0, offdiag=False):
+def random_bse_case(seed, vt=7, ct=33, P=70, rpamin=0, vmin_off=0, offdiag=False, scale=0.3):
     rng = np.random.default_rng(seed)
     ntot = vmin_off + vt + ct
-    M = rng.normal(scale=0.3, size=(ntot, ntot, P))
+    M = rng.normal(scale=scale, size=(ntot, ntot, P))
     M = 0.5 * (M + M.transpose(1, 0, 2))
     eps = rng.uniform(0.1, 1.0, P)
     hs = vt + ct
@@ -163,7 +163,7 @@ def test_bse_operator_random_mixed(ctx, c, prec, tol):
 
 
 def test_davidson_fp64_matches_dense(ctx):
-    M, eps, Hqp, homo, rpamin, vmin, cmax = random_bse_case(20, vt=8, ct=40, P=60)
+    M, eps, Hqp, homo, rpamin, vmin, cmax = random_bse_case(20, vt=8, ct=40, P=60, scale=0.05)
     t = dev_mmn(ctx, M)
     op = px.SingletOperator_TDA(eps, t, Hqp).configure(px.BSEOperator_Options(homo, rpamin, 0, vmin, cmax))
     H = X.bse_dense(1, 2, 1, 0, eps, M, Hqp, homo, rpamin, vmin, cmax)
@@ -181,7 +181,7 @@ def test_davidson_fp64_matches_dense(ctx):
 
 @pytest.mark.parametrize("prec", ["bf16", "bf16x3"])
 def test_davidson_mixed_refined_matches_dense(ctx, prec):
-    M, eps, Hqp, homo, rpamin, vmin, cmax = random_bse_case(21, vt=8, ct=40, P=60)
+    M, eps, Hqp, homo, rpamin, vmin, cmax = random_bse_case(21, vt=8, ct=40, P=60, scale=0.05)
     t = dev_mmn(ctx, M)
     op = px.SingletOperator_TDA(eps, t, Hqp, precisions=("fp64", prec)).configure(
         px.BSEOperator_Options(homo, rpamin, 0, vmin, cmax))
@@ -203,3 +203,89 @@ def test_davidson_forced_failure(ctx):  # test_davidson.cc:107-125
     ds.set_iter_max(1)
     ev, vec = ds.solve(op, 4)
     assert not ds.converged()
+
+
+SIGMA_PPM_E = np.array([-0.612601, -0.341755, -0.341755, -0.341755, 0.137304, 0.16678, 0.16678, 0.16678, 0.671592,
+                        0.671592, 0.671592, 0.974255, 1.01205, 1.01205, 1.01205, 1.64823, 19.4429])
+GW_E = np.array([-10.6784, -0.746424, -0.394948, -0.394948, -0.394948, 0.165212, 0.227713, 0.227713, 0.227713,
+                 0.763971, 0.763971, 0.763971, 1.05054, 1.13372, 1.13372, 1.13372, 1.72964])
+
+
+def ch4_tensor(ctx, ints, mos):
+    S, Saux, V, T = ints
+    C = mmio.read_matrix(os.path.join(G, mos))
+    t = px.TCMatrix_gwbse(ctx).Initialize(17, 0, 16, 0, 16)
+    t.Fill(T, C, X.pseudo_invsqrt_gwbse(V, Saux))
+    return t
+
+
+def test_sigma_ppm_fixtures_gpu(ctx, ints):  # test_sigma_ppm.cc:44-121
+    t = ch4_tensor(ctx, ints, "sigma_ppm/MOs.mm")
+    M = t.get_tensor()
+    e = SIGMA_PPM_E
+    rpa = px.RPA(t)
+    rpa.configure(4, 0, 16)
+    rpa.setRPAInputEnergies(e)
+    sig = px.Sigma_PPM(t, rpa).configure(4, 0, 16, 0, 16, 1e-3)
+    x = sig.CalcExchangeMatrix()
+    assert mmio.isapprox(x, mmio.read_matrix(os.path.join(G, "sigma_ppm/x_ref.mm")), 1e-5)
+    assert relerr(x, X.sigma_x(M, 4, 0, 0, 16)) < 1e-13
+    sig.PrepareScreening()
+    phi, w, f = X.ppm_construct(M, e, 4, 0, 16)
+    assert np.abs(sig.ppm_weight - w).max() < 1e-10
+    assert np.abs(sig.ppm_freq - f).max() < 1e-8
+    c = sig.CalcCorrelationOffDiag(e)
+    c[np.diag_indices(17)] = sig.CalcCorrelationDiag(e)
+    ref = mmio.read_matrix(os.path.join(G, "sigma_ppm/c_ref.mm"))
+    assert mmio.isapprox(np.diag(c), np.diag(ref), 1e-5)
+    assert mmio.isapprox(c, ref, 1e-5)
+    # derivative against a central difference on the device evaluator
+    s, d = sig.diag_batch([3, 3, 3], [e[3] - 1e-5, e[3] + 1e-5, e[3]])
+    assert abs((s[1] - s[0]) / 2e-5 - d[2]) < 1e-5 * max(1.0, abs(d[2]))
+
+
+def test_gw_fixture_gpu(ctx, ints):  # test_gw.cc:111-190
+    t = ch4_tensor(ctx, ints, "gw/mo_eigenvectors.mm")
+    vxc = mmio.read_matrix(os.path.join(G, "gw/vxc.mm"))
+    gw = px.GW(t, vxc, GW_E).configure(4, 0, 16, 0, 16, eta=1e-3, qp_solver="grid",
+                                        full_window_half_width=0.5 * 0.005 * 600, dense_spacing=0.005)
+    gw.CalculateGWPerturbation()
+    ref = mmio.read_matrix(os.path.join(G, "gw/ref.mm"))
+    assert gw.qp_converged.all()
+    assert mmio.isapprox(gw.getGWAResults(), np.diag(ref), 1e-4)
+    gw.CalculateHQP()
+    assert mmio.isapprox(gw.getHQP(), ref, 1e-4)
+
+
+def test_gw_fixed_point_matches_oracle(ctx, ints):
+    t = ch4_tensor(ctx, ints, "gw/mo_eigenvectors.mm")
+    M = t.get_tensor()
+    vxc = mmio.read_matrix(os.path.join(G, "gw/vxc.mm"))
+    e = GW_E
+    gw = px.GW(t, vxc, e).configure(4, 0, 16, 0, 16, eta=1e-3, qp_solver="fixedpoint")
+    gw.CalculateGWPerturbation()
+    sx = X.sigma_x(M, 4, 0, 0, 16)
+    phi, w, f = X.ppm_construct(M, e, 4, 0, 16)
+    Mr = X.multiply_right(M, phi)
+    inter = e + np.diag(sx) - np.diag(vxc)
+    for l in range(17):
+        r = X.qp_fixed_point(lambda x: X.sigma_ppm_diag(Mr, e, w, f, 4, 0, 0, 1e-3, l, x), inter[l], e[l])
+        s = X.sigma_ppm_diag(Mr, e, w, f, 4, 0, 0, 1e-3, l, r)[0]
+        assert abs(gw.getGWAResults()[l] - (sx[l, l] + s - vxc[l, l] + e[l])) < 1e-9
+
+
+def test_setup_screening_gpu(ctx):
+    rng = np.random.default_rng(30)
+    o, v, P = 6, 40, 50
+    M = rng.normal(scale=0.1, size=(o + v, o + v, P))
+    M = 0.5 * (M + M.transpose(1, 0, 2))
+    e = np.concatenate([np.sort(rng.uniform(-1, -0.3, o)), np.sort(rng.uniform(0.1, 2, v))])
+    t = dev_mmn(ctx, M)
+    ei = px.setup_screening(t, e, o - 1, 0, o + v - 1)
+    lam, U = np.linalg.eigh(X.rpa_epsilon(M, e, o - 1, 0, o + v - 1, 0.0, "real"))
+    assert np.abs(ei - 1.0 / lam).max() < 1e-12
+    # the rotated tensor agrees up to the sign of each eigenvector
+    Mr = X.multiply_right(M, U)
+    G_ = t.get_tensor()
+    sgn = np.sign(np.sum(G_ * Mr, axis=(0, 1)))
+    assert relerr(G_ * sgn[None, None, :], Mr) < 1e-10

@@ -156,3 +156,65 @@ def test_quadratures_fixture(scheme, fixture):  # test_gaussian_quadratures.cc:5
     assert mmio.isapprox(vals, ref, 1e-10)
     # closed form: the integral of exp(-x^2) over the real line is sqrt(pi)
     assert abs(vals[-1] - np.sqrt(np.pi)) < 1e-8
+
+
+SIGMA_PPM_E = np.array([-0.612601, -0.341755, -0.341755, -0.341755, 0.137304, 0.16678, 0.16678, 0.16678, 0.671592,
+                        0.671592, 0.671592, 0.974255, 1.01205, 1.01205, 1.01205, 1.64823, 19.4429])
+GW_E = np.array([-10.6784, -0.746424, -0.394948, -0.394948, -0.394948, 0.165212, 0.227713, 0.227713, 0.227713,
+                 0.763971, 0.763971, 0.763971, 1.05054, 1.13372, 1.13372, 1.13372, 1.72964])
+
+
+def test_sigma_ppm_fixtures(ints):  # test_sigma_ppm.cc:44-121
+    M = mmn_for(ints, "sigma_ppm/MOs.mm", 0, 16, 0, 16)
+    e = SIGMA_PPM_E
+    assert mmio.isapprox(X.sigma_x(M, 4, 0, 0, 16), mmio.read_matrix(os.path.join(G, "sigma_ppm/x_ref.mm")), 1e-5)
+    phi, w, f = X.ppm_construct(M, e, 4, 0, 16)
+    Mr = X.multiply_right(M, phi)
+    c = np.zeros((17, 17))
+    for l1 in range(17):
+        for l2 in range(l1 + 1, 17):
+            c[l1, l2] = c[l2, l1] = X.sigma_ppm_offdiag(Mr, e, w, f, 4, 0, 0, 1e-3, l1, l2, e[l1], e[l2])
+        c[l1, l1] = X.sigma_ppm_diag(Mr, e, w, f, 4, 0, 0, 1e-3, l1, e[l1])[0]
+    ref = mmio.read_matrix(os.path.join(G, "sigma_ppm/c_ref.mm"))
+    assert mmio.isapprox(np.diag(c), np.diag(ref), 1e-5)
+    assert mmio.isapprox(c, ref, 1e-5)
+    # the derivative is the derivative: central difference of the diagonal element
+    s_p = X.sigma_ppm_diag(Mr, e, w, f, 4, 0, 0, 1e-3, 3, e[3] + 1e-5)[0]
+    s_m = X.sigma_ppm_diag(Mr, e, w, f, 4, 0, 0, 1e-3, 3, e[3] - 1e-5)[0]
+    ds = X.sigma_ppm_diag(Mr, e, w, f, 4, 0, 0, 1e-3, 3, e[3])[1]
+    assert abs((s_p - s_m) / 2e-5 - ds) < 1e-5 * max(1.0, abs(ds))
+
+
+def test_gw_fixture_grid(ints):  # test_gw.cc:111-190 (G0W0-PPM, grid QP solver, 601 x 0.005 Ha)
+    M = mmn_for(ints, "gw/mo_eigenvectors.mm", 0, 16, 0, 16)
+    vxc = mmio.read_matrix(os.path.join(G, "gw/vxc.mm"))
+    e = GW_E
+    sx = X.sigma_x(M, 4, 0, 0, 16)
+    phi, w, f = X.ppm_construct(M, e, 4, 0, 16)
+    Mr = X.multiply_right(M, phi)
+    inter = e + np.diag(sx) - np.diag(vxc)
+    qp = np.array([X.qp_grid(lambda x: X.sigma_ppm_diag(Mr, e, w, f, 4, 0, 0, 1e-3, l, x), inter[l], e[l], l <= 4,
+                             half_width=0.5 * 0.005 * 600, spacing=0.005) for l in range(17)])
+    scd = np.array([X.sigma_ppm_diag(Mr, e, w, f, 4, 0, 0, 1e-3, l, qp[l])[0] for l in range(17)])
+    gwa = np.diag(sx) + scd - np.diag(vxc) + e
+    ref = mmio.read_matrix(os.path.join(G, "gw/ref.mm"))
+    assert mmio.isapprox(gwa, np.diag(ref), 1e-4)
+    H = sx + np.diag(scd) - vxc + np.diag(e)
+    for l1 in range(17):
+        for l2 in range(17):
+            if l1 != l2:
+                H[l1, l2] += X.sigma_ppm_offdiag(Mr, e, w, f, 4, 0, 0, 1e-3, l1, l2, gwa[l1], gwa[l2])
+    assert mmio.isapprox(H, ref, 1e-4)
+
+
+def test_qp_linearisation_closed_form():  # gw.cc:412-431 with a single-pole Sigma
+    a, b, w0, icpt = 0.02, 0.7, 0.3, 0.25
+    sig = lambda w: (a / (w - b), -a / (w - b) ** 2)
+    s, ds = sig(w0)
+    lin = w0 + (icpt - w0 + s) / (1.0 - ds)
+    root = X.qp_fixed_point(sig, icpt, w0, alpha=1.0, tol=1e-12, max_iter=200)
+    # exact root of w = icpt + a/(w-b): quadratic
+    r = np.roots([1.0, -(icpt + b), icpt * b - a])
+    exact = r[np.argmin(np.abs(r - w0))]
+    assert abs(root - exact) < 1e-10
+    assert abs(lin - exact) < 1e-2  # first-order step lands near the root

@@ -23,7 +23,8 @@ def build(verbose: bool = False) -> str:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     src = os.path.join(CSRC, "xtpgpu.hip")
     tmp = LIB + ".tmp"
-    cmd = [hipcc, *HIPCC_FLAGS, src, "-o", tmp]
+    cmd = [hipcc, *HIPCC_FLAGS, src, "-o", tmp, "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas",
+           "-Wl,-rpath,/opt/rocm/lib"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)

@@ -7,6 +7,8 @@
 //   DavidsonSolver             xtp/include/votca/xtp/davidsonsolver.h:64-280, xtp/src/libxtp/davidsonsolver.cc
 //   OpenMP_CUDA / CudaPipeline xtp/src/libxtp/openmp_cuda.cc, xtp/src/libxtp/cudapipeline.cc
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <chrono>
@@ -181,6 +183,7 @@ extern "C" int xtpgpu_ctx_destroy(xtpgpu_ctx ctx) {
     (void)hipStreamDestroy(ctx->stream);
     ctx->collect();
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+    if (ctx->blas) (void)rocblas_destroy_handle((rocblas_handle)ctx->blas);
     delete ctx;
   });
 }
@@ -998,3 +1001,4 @@ extern "C" int xtpgpu_bse_matmul(xtpgpu_bseop op, int precision, const double* X
 }
 
 #include "davidson.inc"
+#include "gw.inc"

@@ -88,6 +88,7 @@ struct xtpgpu_ctx_s {
   // grow-only scratch buffers, reused across calls on this context
   xg::DevBuf ws[6];
   bool profiling = false;
+  void* blas = nullptr;  // rocblas_handle for the rocSOLVER calls, bound to `stream`
   std::vector<std::pair<std::string, xg::KTimer>> timers;
   std::vector<hipEvent_t> event_pool;
   xg::KTimer& timer(const std::string& n) {

@@ -46,6 +46,14 @@ class DavidsonInfo(C.Structure):
                 ("max_residual", C.c_double), ("seconds", C.c_double)]
 
 
+class QPOpts(C.Structure):
+    _fields_ = [("solver", C.c_int), ("qp_solver_alpha", C.c_double), ("g_sc_limit", C.c_double),
+                ("g_sc_max_iterations", C.c_int), ("full_window_half_width", C.c_double),
+                ("dense_spacing", C.c_double), ("restrict_search", C.c_int), ("zero_margin", C.c_double),
+                ("virtual_min_energy", C.c_double), ("min_accepted_Z", C.c_double),
+                ("max_accepted_Z", C.c_double), ("bisection_max_iter", C.c_int)]
+
+
 # name -> (restype, argtypes): the full exported surface of include/xtpgpu.h
 SIGNATURES = {
     "xtpgpu_last_error": (C.c_char_p, []),
@@ -82,6 +90,15 @@ SIGNATURES = {
     "xtpgpu_bse_matmul_device": (C.c_int, [_vp, C.c_int, _vp, _i64, _i64, _vp, _i64]),
     "xtpgpu_davidson_solve": (C.c_int, [_vp, _i64, C.POINTER(DavidsonOpts), _dp, _dp, _i64,
                                         C.POINTER(DavidsonInfo)]),
+    "xtpgpu_bse_setup_screening": (C.c_int, [_vp, _vp, _dp, _i64, _i64, _i64, C.c_double, _dp]),
+    "xtpgpu_sigma_x": (C.c_int, [_vp, _vp, _i64, _i64, _i64, _i64, _dp]),
+    "xtpgpu_ppm_prepare": (C.c_int, [_vp, _vp, _dp, _i64, _i64, _i64, _dp, _dp]),
+    "xtpgpu_sigma_ppm_create": (C.c_int, [_vp, _vp, _dp, _dp, _dp, _i64, _i64, _i64, C.c_double,
+                                          C.POINTER(_vp)]),
+    "xtpgpu_sigma_destroy": (C.c_int, [_vp]),
+    "xtpgpu_sigma_ppm_diag": (C.c_int, [_vp, _i64, C.POINTER(_i64), _dp, _dp, _dp]),
+    "xtpgpu_sigma_ppm_offdiag": (C.c_int, [_vp, _i64, _dp, _dp]),
+    "xtpgpu_gw_solve_qp": (C.c_int, [_vp, _i64, _dp, _dp, C.POINTER(QPOpts), _dp, C.POINTER(C.c_int)]),
 }
 
 _lib = None
@@ -484,3 +501,134 @@ class DavidsonSolver:
 
     def converged(self):
         return self._converged
+
+
+def setup_screening(Mmn: TCMatrix_gwbse, energies, homo, rpamin, rpamax, energy=0.0) -> np.ndarray:
+    """BSE::SetupDirectInteractionOperator (bse.cc:188-204): rotates Mmn in place, returns eps0^-1."""
+    e = np.ascontiguousarray(energies, dtype=np.float64)
+    out = np.zeros(Mmn.aux)
+    _check(lib().xtpgpu_bse_setup_screening(Mmn.ctx.h, Mmn.h, _dptr(e), homo, rpamin, rpamax, energy, _dptr(out)))
+    return out
+
+
+class Sigma_PPM:
+    """Sigma_PPM (sigma_ppm.h) + Sigma_base exchange (sigma_base.h:33-106) on the device."""
+
+    def __init__(self, Mmn: TCMatrix_gwbse, rpa: RPA):
+        self.Mmn, self.rpa, self.h = Mmn, rpa, None
+
+    def configure(self, homo, qpmin, qpmax, rpamin, rpamax, eta=1e-3):
+        self.homo, self.qpmin, self.qpmax, self.rpamin, self.rpamax, self.eta = homo, qpmin, qpmax, rpamin, rpamax, eta
+        self.qptotal = qpmax - qpmin + 1
+        return self
+
+    def CalcExchangeMatrix(self) -> np.ndarray:
+        out = np.zeros((self.qptotal, self.qptotal), order="F")
+        _check(lib().xtpgpu_sigma_x(self.Mmn.ctx.h, self.Mmn.h, self.homo, self.rpamin, self.qpmin, self.qpmax,
+                                    _dptr(out)))
+        return out
+
+    def PrepareScreening(self):
+        P = self.Mmn.aux
+        self.ppm_weight, self.ppm_freq = np.zeros(P), np.zeros(P)
+        e = np.ascontiguousarray(self.rpa.getRPAInputEnergies())
+        _check(lib().xtpgpu_ppm_prepare(self.Mmn.ctx.h, self.Mmn.h, _dptr(e), self.homo, self.rpamin, self.rpamax,
+                                        _dptr(self.ppm_weight), _dptr(self.ppm_freq)))
+        self._bind()
+
+    def _bind(self):
+        self.close()
+        h = _vp()
+        e = np.ascontiguousarray(self.rpa.getRPAInputEnergies())
+        _check(lib().xtpgpu_sigma_ppm_create(self.Mmn.ctx.h, self.Mmn.h, _dptr(e), _dptr(self.ppm_weight),
+                                             _dptr(self.ppm_freq), self.homo, self.rpamin, self.qpmin, self.eta,
+                                             C.byref(h)))
+        self.h = h
+
+    def diag_batch(self, levels, omegas):
+        lv = np.ascontiguousarray(levels, dtype=np.int64)
+        om = np.ascontiguousarray(omegas, dtype=np.float64)
+        s, d = np.zeros(len(om)), np.zeros(len(om))
+        _check(lib().xtpgpu_sigma_ppm_diag(self.h, len(om), lv.ctypes.data_as(C.POINTER(_i64)), _dptr(om), _dptr(s),
+                                           _dptr(d)))
+        return s, d
+
+    def CalcCorrelationDiagElement(self, gw_level, frequency):
+        return float(self.diag_batch([gw_level], [frequency])[0][0])
+
+    def CalcCorrelationDiagElementDerivative(self, gw_level, frequency):
+        return float(self.diag_batch([gw_level], [frequency])[1][0])
+
+    def CalcCorrelationDiag(self, frequencies):
+        return self.diag_batch(np.arange(self.qptotal), frequencies)[0]
+
+    def CalcCorrelationOffDiag(self, frequencies):
+        f = np.ascontiguousarray(frequencies, dtype=np.float64)
+        out = np.zeros((self.qptotal, self.qptotal), order="F")
+        _check(lib().xtpgpu_sigma_ppm_offdiag(self.h, self.qptotal, _dptr(f), _dptr(out)))
+        return out
+
+    def SolveQP(self, intercepts, frequencies, solver="fixedpoint", **kw):
+        o = QPOpts(0 if solver == "fixedpoint" else 1, kw.get("qp_solver_alpha", 0.75), kw.get("g_sc_limit", 1e-5),
+                   kw.get("g_sc_max_iterations", 50), kw.get("full_window_half_width", 0.75),
+                   kw.get("dense_spacing", 0.002), 1 if kw.get("restrict_search", True) else 0,
+                   kw.get("zero_margin", 1e-6), kw.get("virtual_min_energy", -0.1), kw.get("min_accepted_Z", 0.05),
+                   kw.get("max_accepted_Z", 1.5), kw.get("bisection_max_iter", 200))
+        ic = np.ascontiguousarray(intercepts, dtype=np.float64)
+        fr = np.ascontiguousarray(frequencies, dtype=np.float64)
+        out = np.zeros(self.qptotal)
+        conv = (C.c_int * self.qptotal)()
+        _check(lib().xtpgpu_gw_solve_qp(self.h, self.qptotal, _dptr(ic), _dptr(fr), C.byref(o), _dptr(out), conv))
+        return out, np.array(list(conv), dtype=bool)
+
+    def close(self):
+        if self.h:
+            lib().xtpgpu_sigma_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GW:
+    """GW::configure / CalculateGWPerturbation / CalculateHQP / getHQP (gw.cc:35-410, 772-776) for
+    G0W0 with the plasmon-pole self-energy, driving the device kernels."""
+
+    def __init__(self, Mmn: TCMatrix_gwbse, vxc, dft_energies):
+        self.Mmn = Mmn
+        self.vxc = np.asarray(vxc, dtype=np.float64)
+        self.dft = np.asarray(dft_energies, dtype=np.float64)
+
+    def configure(self, homo, qpmin, qpmax, rpamin, rpamax, eta=1e-3, ScaHFX=0.0, qp_solver="grid", **qp):
+        self.homo, self.qpmin, self.qpmax, self.rpamin, self.rpamax = homo, qpmin, qpmax, rpamin, rpamax
+        self.qptotal = qpmax - qpmin + 1
+        self.ScaHFX, self.qp_solver, self.qp = ScaHFX, qp_solver, qp
+        self.rpa = RPA(self.Mmn)
+        self.rpa.configure(homo, rpamin, rpamax)
+        self.sigma = Sigma_PPM(self.Mmn, self.rpa).configure(homo, qpmin, qpmax, rpamin, rpamax, eta)
+        return self
+
+    def CalculateGWPerturbation(self):
+        self.Sigma_x = (1.0 - self.ScaHFX) * self.sigma.CalcExchangeMatrix()
+        self.rpa.setRPAInputEnergies(self.dft[self.rpamin:self.rpamax + 1])
+        freqs = self.dft[self.qpmin:self.qpmax + 1].copy()
+        self.sigma.PrepareScreening()
+        intercepts = self.dft[self.qpmin:self.qpmax + 1] + np.diag(self.Sigma_x) - np.diag(self.vxc)
+        freqs, self.qp_converged = self.sigma.SolveQP(intercepts, freqs, self.qp_solver, **self.qp)
+        self.Sigma_c = np.zeros((self.qptotal, self.qptotal))
+        self.Sigma_c[np.diag_indices(self.qptotal)] = self.sigma.CalcCorrelationDiag(freqs)
+        return self
+
+    def getGWAResults(self):
+        return np.diag(self.Sigma_x) + np.diag(self.Sigma_c) - np.diag(self.vxc) + self.dft[self.qpmin:self.qpmax + 1]
+
+    def CalculateHQP(self):
+        d = np.diag(self.Sigma_c).copy()
+        self.Sigma_c = self.sigma.CalcCorrelationOffDiag(self.getGWAResults())
+        self.Sigma_c[np.diag_indices(self.qptotal)] = d
+
+    def getHQP(self):
+        return self.Sigma_x + self.Sigma_c - self.vxc + np.diag(self.dft[self.qpmin:self.qpmax + 1])
