@@ -216,6 +216,30 @@ typedef struct {
 int xtpgpu_davidson_solve(xtpgpu_bseop op, int64_t neigen, const xtpgpu_davidson_opts* opts,
                           double* evals, double* evecs, int64_t ldv, xtpgpu_davidson_info* info);
 
+/* Full BSE: DavidsonSolver::solve on HamiltonianOperator<A,B> = [[A, B], [-B, -A]]
+ * (bseoperator_btda.h:44-150) with the HAM matrix type (harmonic Ritz, davidsonsolver.cc:241-332),
+ * the X-ranked initial guess (bse_initialization.h:47-93) and the X/Y normalisation of
+ * BSE::Solve_nonhermitian_Davidson (bse.cc:320-358). A = singlet/triplet TDA operator, B = the
+ * matching coupling operator (cqp=0, cx=2/0, cd=0, cd2=1). X, Y: size x neigen (may be null). */
+int xtpgpu_davidson_solve_full(xtpgpu_bseop A, xtpgpu_bseop B, int64_t neigen,
+                               const xtpgpu_davidson_opts* opts, double* evals, double* X,
+                               double* Y, int64_t ldv, xtpgpu_davidson_info* info);
+
+/* ---------------------------------------------------------------- adjoint */
+/* Hellmann-Feynman derivatives of converged BSE roots at fixed screening, the contraction
+ * pattern of BSE::ExpectationValue_Operator (bse.cc:490-606): dOmega = X^T dA X + Y^T dA Y +
+ * 2 X^T dB Y. B and Y null for TDA. X, Y: size x nroots (host, leading dim ldv), normalised as
+ * the solvers return them. d_eps (nullable): (vtotal+ctotal) x nroots, dOmega/de for the BSE
+ * levels (occupied first); d_m (nullable): auxsize x vtotal x ctotal x nroots, dOmega/dM^P_{vc}
+ * (element (P,v,c) of root r at r*P*N + P*N + v*ctotal + c) for the symmetric transition block
+ * M^P_{vc} = M^P_{cv}. No reference counterpart computes these; the reference obtains
+ * excited-state derivatives by finite differences (xtp/src/pyxtp/pyxtp/xtp.py:592-682). */
+int xtpgpu_bse_adjoint(xtpgpu_bseop A, xtpgpu_bseop B, int64_t nroots, const double* X,
+                       const double* Y, int64_t ldv, double* d_eps, double* d_m);
+/* Same with d_m a device buffer (stays resident for device-side consumers such as autograd). */
+int xtpgpu_bse_adjoint_device(xtpgpu_bseop A, xtpgpu_bseop B, int64_t nroots, const double* X,
+                              const double* Y, int64_t ldv, double* d_eps, double* d_m_dev);
+
 #ifdef __cplusplus
 }
 #endif

@@ -289,3 +289,81 @@ def test_setup_screening_gpu(ctx):
     G_ = t.get_tensor()
     sgn = np.sign(np.sum(G_ * Mr, axis=(0, 1)))
     assert relerr(G_ * sgn[None, None, :], Mr) < 1e-10
+
+
+def test_full_bse_davidson_matches_dense(ctx):
+    M, eps, Hqp, homo, rpamin, vmin, cmax = random_bse_case(40, vt=6, ct=30, P=50, scale=0.05)
+    t = dev_mmn(ctx, M)
+    opt = px.BSEOperator_Options(homo, rpamin, 0, vmin, cmax)
+    A = px.SingletOperator_TDA(eps, t, Hqp).configure(opt)
+    B = px.SingletOperator_BTDA_B(eps, t, Hqp).configure(opt)
+    Ad = X.bse_dense(1, 2, 1, 0, eps, M, Hqp, homo, rpamin, vmin, cmax)
+    Bd = X.bse_dense(0, 2, 0, 1, eps, M, Hqp, homo, rpamin, vmin, cmax)
+    H = np.block([[Ad, Bd], [-Bd, -Ad]])
+    w = np.sort(np.linalg.eigvals(H).real)
+    w = w[w > 0][:4]
+    # closed form for real symmetric A, B: Omega^2 = eig((A-B)^1/2 (A+B) (A-B)^1/2)
+    lam, U = np.linalg.eigh(Ad - Bd)
+    S = (U * np.sqrt(lam)) @ U.T
+    w2 = np.sqrt(np.linalg.eigvalsh(S @ (Ad + Bd) @ S))[:4]
+    assert np.abs(w - w2).max() < 1e-10
+    ds = px.DavidsonSolver()
+    ds.set_tolerance(1e-8)
+    ev, Xv, Yv = ds.solve_full(A, B, 4)
+    assert ds.converged(), ds.info.max_residual
+    assert np.abs(ev - w).max() < 1e-7
+    # X/Y normalisation |X|^2 - |Y|^2 = 1 and the eigen-equation
+    assert np.abs(np.sum(Xv ** 2, 0) - np.sum(Yv ** 2, 0) - 1).max() < 1e-10
+    r = Ad @ Xv + Bd @ Yv - Xv * ev
+    assert np.abs(r).max() < 1e-6
+
+
+def _perturb_sym(M, P, v, c, vmin_r, cmin_r, h):
+    Mp = M.copy()
+    Mp[vmin_r + v, cmin_r + c, P] += h
+    Mp[cmin_r + c, vmin_r + v, P] += h
+    return Mp
+
+
+@pytest.mark.parametrize("full", [False, True])
+def test_adjoint_matches_finite_differences(ctx, full):
+    M, eps, Hqp, homo, rpamin, vmin, cmax = random_bse_case(50, vt=5, ct=22, P=30, scale=0.05)
+    vt, ct = homo - vmin + 1, cmax - homo
+    t = dev_mmn(ctx, M)
+    opt = px.BSEOperator_Options(homo, rpamin, 0, vmin, cmax)
+    A = px.SingletOperator_TDA(eps, t, Hqp).configure(opt)
+    B = px.SingletOperator_BTDA_B(eps, t, Hqp).configure(opt) if full else None
+    ds = px.DavidsonSolver()
+    ds.set_tolerance(1e-10)
+    nr = 3
+    if full:
+        ev, Xv, Yv = ds.solve_full(A, B, nr)
+    else:
+        ev, Xv = ds.solve(A, nr)
+        Yv = None
+    assert ds.converged()
+    d_eps, d_m = px.bse_adjoint(A, Xv, B, Yv)
+
+    def omegas(Mx, H):
+        Ad = X.bse_dense(1, 2, 1, 0, eps, Mx, H, homo, rpamin, vmin, cmax)
+        if not full:
+            return np.linalg.eigvalsh(Ad)[:nr]
+        Bd = X.bse_dense(0, 2, 0, 1, eps, Mx, H, homo, rpamin, vmin, cmax)
+        w = np.sort(np.linalg.eigvals(np.block([[Ad, Bd], [-Bd, -Ad]])).real)
+        return w[w > 0][:nr]
+
+    assert np.abs(omegas(M, Hqp) - ev).max() < 1e-8
+    h = 1e-5
+    rng = np.random.default_rng(51)
+    for lvl in rng.choice(vt + ct, 6, replace=False):
+        Hp, Hm = Hqp.copy(), Hqp.copy()
+        Hp[lvl, lvl] += h
+        Hm[lvl, lvl] -= h
+        fd = (omegas(M, Hp) - omegas(M, Hm)) / (2 * h)
+        assert np.abs(fd - d_eps[lvl]).max() < 1e-5 * max(1.0, np.abs(fd).max()), (lvl, fd, d_eps[lvl])
+    for _ in range(6):
+        P, v, c = rng.integers(30), rng.integers(vt), rng.integers(ct)
+        fd = (omegas(_perturb_sym(M, P, v, c, vmin - rpamin, homo + 1 - rpamin, h), Hqp)
+              - omegas(_perturb_sym(M, P, v, c, vmin - rpamin, homo + 1 - rpamin, -h), Hqp)) / (2 * h)
+        an = d_m[:, P, v, c]
+        assert np.abs(fd - an).max() < 1e-5 * max(1e-3, np.abs(fd).max()), (P, v, c, fd, an)

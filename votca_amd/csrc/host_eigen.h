@@ -120,3 +120,179 @@ inline bool sym_eig(int n, const double* A, int lda, double* w, double* Z) {
 }
 
 }  // namespace xg
+
+namespace xg {
+
+// Real nonsymmetric eigenvalues: Householder reduction to upper Hessenberg form, then the
+// Francis implicit double-shift QR iteration (Golub & Van Loan, Alg. 7.4.2 and 7.5.1-7.5.2).
+// A: n x n column-major (copied). wr/wi receive real and imaginary parts. Returns false if
+// the iteration did not converge.
+inline bool nonsym_eigvals(int n, const double* A, int lda, double* wr, double* wi) {
+  std::vector<double> h((size_t)n * n);
+  auto H = [&](int i, int j) -> double& { return h[(size_t)j * n + i]; };
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) H(i, j) = A[(size_t)j * lda + i];
+  std::vector<double> v(n);
+  for (int k = 0; k + 2 < n; ++k) {
+    double alpha = 0.0;
+    for (int i = k + 1; i < n; ++i) alpha += H(i, k) * H(i, k);
+    alpha = std::sqrt(alpha);
+    if (alpha == 0.0) continue;
+    if (H(k + 1, k) > 0) alpha = -alpha;
+    for (int i = 0; i < n; ++i) v[i] = 0.0;
+    for (int i = k + 1; i < n; ++i) v[i] = H(i, k);
+    v[k + 1] -= alpha;
+    double vn = 0.0;
+    for (int i = k + 1; i < n; ++i) vn += v[i] * v[i];
+    if (vn == 0.0) continue;
+    const double beta = 2.0 / vn;
+    for (int j = k; j < n; ++j) {  // H <- (I - b v v^T) H
+      double s = 0.0;
+      for (int i = k + 1; i < n; ++i) s += v[i] * H(i, j);
+      s *= beta;
+      for (int i = k + 1; i < n; ++i) H(i, j) -= s * v[i];
+    }
+    for (int i = 0; i < n; ++i) {  // H <- H (I - b v v^T)
+      double s = 0.0;
+      for (int j = k + 1; j < n; ++j) s += H(i, j) * v[j];
+      s *= beta;
+      for (int j = k + 1; j < n; ++j) H(i, j) -= s * v[j];
+    }
+    for (int i = k + 2; i < n; ++i) H(i, k) = 0.0;
+  }
+  const double eps = 2.220446049250313e-16;
+  int hi = n - 1, iter = 0;
+  while (hi >= 0) {
+    if (hi == 0) { wr[0] = H(0, 0); wi[0] = 0.0; --hi; continue; }
+    // find the start of the active unreduced block
+    int l = hi;
+    while (l > 0) {
+      double s = std::fabs(H(l - 1, l - 1)) + std::fabs(H(l, l));
+      if (s == 0.0) s = 1.0;
+      if (std::fabs(H(l, l - 1)) < eps * s) { H(l, l - 1) = 0.0; break; }
+      --l;
+    }
+    if (l == hi) { wr[hi] = H(hi, hi); wi[hi] = 0.0; --hi; iter = 0; continue; }
+    if (l == hi - 1) {  // 2x2 block
+      double a = H(hi - 1, hi - 1), b = H(hi - 1, hi), c = H(hi, hi - 1), d = H(hi, hi);
+      double tr = 0.5 * (a + d), det = a * d - b * c, disc = tr * tr - det;
+      if (disc >= 0.0) {
+        double r = std::sqrt(disc);
+        double e1 = tr + (tr >= 0 ? r : -r);
+        double e2 = e1 != 0.0 ? det / e1 : tr - r;
+        wr[hi - 1] = e1; wi[hi - 1] = 0.0;
+        wr[hi] = e2; wi[hi] = 0.0;
+      } else {
+        double r = std::sqrt(-disc);
+        wr[hi - 1] = tr; wi[hi - 1] = r;
+        wr[hi] = tr; wi[hi] = -r;
+      }
+      hi -= 2;
+      iter = 0;
+      continue;
+    }
+    if (++iter > 60 * n) return false;
+    // Francis double shift on rows/cols l..hi
+    double s, t;
+    if (iter % 11 == 10) {  // exceptional shift
+      double w = std::fabs(H(hi, hi - 1)) + std::fabs(H(hi - 1, hi - 2));
+      s = 1.5 * w;
+      t = w * w;
+    } else {
+      s = H(hi - 1, hi - 1) + H(hi, hi);
+      t = H(hi - 1, hi - 1) * H(hi, hi) - H(hi - 1, hi) * H(hi, hi - 1);
+    }
+    double x = H(l, l) * H(l, l) + H(l, l + 1) * H(l + 1, l) - s * H(l, l) + t;
+    double y = H(l + 1, l) * (H(l, l) + H(l + 1, l + 1) - s);
+    double z = (l + 2 <= hi) ? H(l + 2, l + 1) * H(l + 1, l) : 0.0;
+    for (int k = l - 1; k <= hi - 3; ++k) {
+      // reflector P for (x, y, z) acting on rows k+1..k+3
+      double u[3] = {x, y, z};
+      double al = std::sqrt(x * x + y * y + z * z);
+      if (al != 0.0) {
+        if (x > 0) al = -al;
+        u[0] -= al;
+        double un = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        if (un != 0.0) {
+          double be = 2.0 / un;
+          int q = std::max(k, l);
+          for (int j = q; j < n; ++j) {
+            double d0 = be * (u[0] * H(k + 1, j) + u[1] * H(k + 2, j) + u[2] * H(k + 3, j));
+            H(k + 1, j) -= d0 * u[0]; H(k + 2, j) -= d0 * u[1]; H(k + 3, j) -= d0 * u[2];
+          }
+          int r = std::min(k + 4, hi);
+          for (int i = 0; i <= r; ++i) {
+            double d0 = be * (u[0] * H(i, k + 1) + u[1] * H(i, k + 2) + u[2] * H(i, k + 3));
+            H(i, k + 1) -= d0 * u[0]; H(i, k + 2) -= d0 * u[1]; H(i, k + 3) -= d0 * u[2];
+          }
+        }
+      }
+      x = H(k + 2, k + 1);
+      y = H(k + 3, k + 1);
+      if (k < hi - 3) z = H(k + 4, k + 1);
+    }
+    {  // final 2-element reflector on rows hi-1, hi
+      double u[2] = {x, y};
+      double al = std::sqrt(x * x + y * y);
+      if (al != 0.0) {
+        if (x > 0) al = -al;
+        u[0] -= al;
+        double un = u[0] * u[0] + u[1] * u[1];
+        if (un != 0.0) {
+          double be = 2.0 / un;
+          for (int j = hi - 2 >= 0 ? hi - 2 : 0; j < n; ++j) {
+            double d0 = be * (u[0] * H(hi - 1, j) + u[1] * H(hi, j));
+            H(hi - 1, j) -= d0 * u[0]; H(hi, j) -= d0 * u[1];
+          }
+          for (int i = 0; i <= hi; ++i) {
+            double d0 = be * (u[0] * H(i, hi - 1) + u[1] * H(i, hi));
+            H(i, hi - 1) -= d0 * u[0]; H(i, hi) -= d0 * u[1];
+          }
+        }
+      }
+    }
+  }
+  return true;
+}
+
+// Right eigenvector of A for a real eigenvalue lam by inverse iteration on (A - lam' I),
+// lam' perturbed by a relative 1e-10 so the LU factorization (partial pivoting) stays finite.
+inline void real_eigvec(int n, const double* A, int lda, double lam, double* x) {
+  std::vector<double> M((size_t)n * n);
+  const double shift = lam + 1e-10 * std::max(1.0, std::fabs(lam));
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) M[(size_t)j * n + i] = A[(size_t)j * lda + i] - (i == j ? shift : 0.0);
+  std::vector<int> piv(n);
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    for (int i = k + 1; i < n; ++i)
+      if (std::fabs(M[(size_t)k * n + i]) > std::fabs(M[(size_t)k * n + p])) p = i;
+    piv[k] = p;
+    if (p != k)
+      for (int j = 0; j < n; ++j) std::swap(M[(size_t)j * n + k], M[(size_t)j * n + p]);
+    double d = M[(size_t)k * n + k];
+    if (d == 0.0) d = M[(size_t)k * n + k] = 1e-300;
+    for (int i = k + 1; i < n; ++i) M[(size_t)k * n + i] /= d;
+    for (int j = k + 1; j < n; ++j) {
+      double f = M[(size_t)j * n + k];
+      if (f != 0.0)
+        for (int i = k + 1; i < n; ++i) M[(size_t)j * n + i] -= M[(size_t)k * n + i] * f;
+    }
+  }
+  for (int i = 0; i < n; ++i) x[i] = 1.0 / std::sqrt((double)n) * (1.0 + 0.01 * (double)(i % 7));
+  for (int it = 0; it < 3; ++it) {
+    for (int k = 0; k < n; ++k) std::swap(x[k], x[piv[k]]);
+    for (int j = 0; j < n; ++j)
+      for (int i = j + 1; i < n; ++i) x[i] -= M[(size_t)j * n + i] * x[j];
+    for (int j = n - 1; j >= 0; --j) {
+      x[j] /= M[(size_t)j * n + j];
+      for (int i = 0; i < j; ++i) x[i] -= M[(size_t)j * n + i] * x[j];
+    }
+    double nr = 0.0;
+    for (int i = 0; i < n; ++i) nr += x[i] * x[i];
+    nr = std::sqrt(nr);
+    for (int i = 0; i < n; ++i) x[i] /= nr;
+  }
+}
+
+}  // namespace xg

@@ -1001,4 +1001,6 @@ extern "C" int xtpgpu_bse_matmul(xtpgpu_bseop op, int precision, const double* X
 }
 
 #include "davidson.inc"
+#include "davidson_ham.inc"
+#include "adjoint.inc"
 #include "gw.inc"

@@ -90,6 +90,10 @@ SIGNATURES = {
     "xtpgpu_bse_matmul_device": (C.c_int, [_vp, C.c_int, _vp, _i64, _i64, _vp, _i64]),
     "xtpgpu_davidson_solve": (C.c_int, [_vp, _i64, C.POINTER(DavidsonOpts), _dp, _dp, _i64,
                                         C.POINTER(DavidsonInfo)]),
+    "xtpgpu_davidson_solve_full": (C.c_int, [_vp, _vp, _i64, C.POINTER(DavidsonOpts), _dp, _dp, _dp, _i64,
+                                             C.POINTER(DavidsonInfo)]),
+    "xtpgpu_bse_adjoint": (C.c_int, [_vp, _vp, _i64, _dp, _dp, _i64, _dp, _dp]),
+    "xtpgpu_bse_adjoint_device": (C.c_int, [_vp, _vp, _i64, _dp, _dp, _i64, _dp, _vp]),
     "xtpgpu_bse_setup_screening": (C.c_int, [_vp, _vp, _dp, _i64, _i64, _i64, C.c_double, _dp]),
     "xtpgpu_sigma_x": (C.c_int, [_vp, _vp, _i64, _i64, _i64, _i64, _dp]),
     "xtpgpu_ppm_prepare": (C.c_int, [_vp, _vp, _dp, _i64, _i64, _i64, _dp, _dp]),
@@ -375,6 +379,8 @@ class BSE_OPERATOR:
         n = _i64(0)
         _check(lib().xtpgpu_bse_size(h, C.byref(n)))
         self.size = n.value
+        self.vt = opt.homo - opt.vmin + 1
+        self.ct = opt.cmax - opt.homo
         return self
 
     def rows(self):
@@ -489,6 +495,20 @@ class DavidsonSolver:
         if rc not in (0, -4):
             _check(rc)
         return ev, vec
+
+    def solve_full(self, A: BSE_OPERATOR, B: BSE_OPERATOR, neigen: int, want_vectors=True):
+        """Full BSE with the HamiltonianOperator [[A,B],[-B,-A]] (BSE::Solve_nonhermitian_Davidson)."""
+        ev = np.zeros(neigen)
+        Xv = np.zeros((A.size, neigen), order="F") if want_vectors else None
+        Yv = np.zeros((A.size, neigen), order="F") if want_vectors else None
+        rc = lib().xtpgpu_davidson_solve_full(A.h, B.h, neigen, C.byref(self.opts), _dptr(ev),
+                                              _dptr(Xv) if want_vectors else None,
+                                              _dptr(Yv) if want_vectors else None, A.size, C.byref(self.info))
+        self._eval, self._evec, self._evec2 = ev, Xv, Yv
+        self._converged = rc == 0
+        if rc not in (0, -4):
+            _check(rc)
+        return ev, Xv, Yv
 
     def eigenvalues(self):
         return self._eval
@@ -632,3 +652,18 @@ class GW:
 
     def getHQP(self):
         return self.Sigma_x + self.Sigma_c - self.vxc + np.diag(self.dft[self.qpmin:self.qpmax + 1])
+
+
+def bse_adjoint(A: BSE_OPERATOR, X, B: BSE_OPERATOR | None = None, Y=None, want_m=True):
+    """Hellmann-Feynman derivatives of BSE roots (include/xtpgpu.h xtpgpu_bse_adjoint).
+    Returns (d_eps[(vt+ct), nroots], d_m[nroots, P, vt, ct] or None)."""
+    Xf = _f64(X)
+    nroots = Xf.shape[1]
+    Yf = _f64(Y) if Y is not None else None
+    vt, ct = A.vt, A.ct
+    d_eps = np.zeros((vt + ct, nroots), order="F")
+    d_m = np.zeros((nroots, A.Mmn.aux, vt, ct)) if want_m else None
+    _check(lib().xtpgpu_bse_adjoint(A.h, B.h if B is not None else None, nroots, _dptr(Xf),
+                                    _dptr(Yf) if Yf is not None else None, Xf.shape[0], _dptr(d_eps),
+                                    _dptr(d_m) if want_m else None))
+    return d_eps, d_m
