@@ -93,6 +93,12 @@ int xtpgpu_mmn_fill_ao2mo(xtpgpu_mmn mmn, const double* ao3c, int64_t nbf, const
  * host restatement reproduces it bit for bit); symmetric in (m,n) as a three-centre quantity is. */
 int xtpgpu_mmn_fill_synthetic(xtpgpu_mmn mmn, uint64_t seed, double sigma0, double decay);
 int xtpgpu_mmn_sizes(xtpgpu_mmn mmn, int64_t* mtotal, int64_t* ntotal, int64_t* auxsize);
+/* Write a transition block: Mmn_[v](c, P) = m[(P*vtotal + v)*ctotal + c] for slices v in
+ * [vmin, vmin+vtotal) and rows c in [cmin, cmin+ctotal) (indices relative to mmin / nmin), and
+ * with symmetric != 0 also Mmn_[c](v, P), as a three-centre tensor is symmetric in (m, n).
+ * m is a host buffer (auxsize x vtotal x ctotal). */
+int xtpgpu_mmn_set_block(xtpgpu_mmn mmn, int64_t vmin, int64_t vtotal, int64_t cmin, int64_t ctotal,
+                         const double* m, int symmetric);
 
 /* ---------------------------------------------------------------- RPA */
 /* RPA::calculate_epsilon<imag> and calculate_epsilon_r(complex) (rpa.cc:75-202):
@@ -161,6 +167,23 @@ int xtpgpu_sigma_ppm_diag(xtpgpu_sigma s, int64_t n, const int64_t* gw_levels, c
 /* Sigma_base::CalcCorrelationOffDiag with Sigma_PPM::CalcCorrelationOffDiagElement
  * (sigma_base.cc:65-78, sigma_ppm.cc:91-126); the diagonal of out is left zero. */
 int xtpgpu_sigma_ppm_offdiag(xtpgpu_sigma s, int64_t qptotal, const double* frequencies, double* out);
+
+/* Sigma_CDA (sigma_cda.cc:30-141) with ImaginaryAxisIntegration (ImaginaryAxisIntegration.cc:90-176):
+ * PrepareScreening at create time -- kappa0 = eps_r(0)^-1 - 1 and D_j = -(eps_i(x_j)^-1 - 1) +
+ * kappa0 exp(-(alpha x_j)^2) on the device -- with the frequency-independent contractions
+ * S_{j,l,n} = (M_l D_j M_l^T)_nn and T_{l,n} = (M_l kappa0 M_l^T)_nn done once. points/weights are
+ * the scaled quadrature nodes and weights (QuadratureFactory scheme and order), symmetry the
+ * scheme's UseSymmetry(). The tensor must span the RPA window in n. */
+typedef struct xtpgpu_cda_s* xtpgpu_cda;
+int xtpgpu_sigma_cda_create(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* rpa_energies, int64_t homo,
+                            int64_t rpamin, int64_t rpamax, int64_t qpmin, int64_t qpmax, double eta,
+                            double alpha, const double* points, const double* weights, int64_t order,
+                            int symmetry, xtpgpu_cda* out);
+int xtpgpu_sigma_cda_destroy(xtpgpu_cda c);
+/* CalcCorrelationDiagElement for n (gw_level, omega) pairs (residues + tail + imaginary-axis
+ * integral); dsigma (nullable) by the reference's central difference, h = 1e-3 (sigma_cda.h:56-62). */
+int xtpgpu_sigma_cda_diag(xtpgpu_cda c, int64_t n, const int64_t* gw_levels, const double* omegas,
+                          double* sigma, double* dsigma);
 
 typedef struct {
   int solver;                    /* 0 = fixedpoint (then grid, then linearisation); 1 = grid (then linearisation) */

@@ -545,3 +545,79 @@ def qp_grid(sigma_fn, intercept, omega0, occupied, half_width=0.75, spacing=0.00
         if score > best_score:
             best, best_score = w, score
     return best
+
+
+# ---------------------------------------------------------------- Sigma_c by contour deformation
+class SigmaCDAOracle:
+    """Sigma_CDA (xtp/src/libxtp/self_energy_evaluators/sigma_cda.cc:30-141) with
+    ImaginaryAxisIntegration (xtp/src/libxtp/ImaginaryAxisIntegration.cc:90-176)."""
+
+    def __init__(self, M, energies, homo, rpamin, rpamax, qpmin, eta, alpha, points, weights, symmetry):
+        self.M, self.e = M, np.asarray(energies, dtype=np.float64)
+        self.homo, self.rpamin, self.rpamax, self.qpmin = homo, rpamin, rpamax, qpmin
+        self.eta, self.alpha = eta, alpha
+        self.points, self.weights, self.symmetry = np.asarray(points), np.asarray(weights), symmetry
+        P = M.shape[2]
+        # PrepareScreening: kappa0 = eps_r(0+0i)^-1 - 1  (sigma_cda.cc:30-45)
+        self.k0 = np.linalg.inv(rpa_epsilon(M, self.e, homo, rpamin, rpamax, 0.0 + 0.0j, "complex")) - np.eye(P)
+        # CalcDielInvVector (ImaginaryAxisIntegration.cc:90-101)
+        self.D = []
+        for x in self.points:
+            einv = np.linalg.inv(rpa_epsilon(M, self.e, homo, rpamin, rpamax, x, "imag")) - np.eye(P)
+            self.D.append(-einv + self.k0 * np.exp(-(alpha * x) ** 2))
+
+    def sigma_gq(self, omega, level):
+        """ImaginaryAxisIntegration::SigmaGQDiag (ImaginaryAxisIntegration.cc:140-176)."""
+        occ = self.homo + 1 - self.rpamin
+        Imx = self.M[level + self.qpmin - self.rpamin]
+        dE = (omega - self.e).astype(np.complex128)
+        dE[:occ] += 1j * self.eta
+        dE[occ:] -= 1j * self.eta
+        tot = 0.0
+        for j, (x, w) in enumerate(zip(self.points, self.weights)):
+            if self.symmetry:
+                den = 1.0 / (dE + 1j * x) + 1.0 / (dE - 1j * x)
+                f = 2.0
+            else:
+                den = 1.0 / (dE + 1j * x)
+                f = 1.0
+            S = np.einsum("np,pq,nq->n", Imx, self.D[j], Imx)
+            # GaussianQuadratureBase::Integrate multiplies by the symmetry factor only through the
+            # integrand; FunctionEvaluation itself returns 0.5/pi Re sum(...)
+            tot += w * 0.5 / np.pi * float(np.sum(S * den).real)
+        return tot
+
+    @staticmethod
+    def prefactor(e_f, e_m, w, tol=1e-10):
+        """Sigma_CDA::CalcResiduePrefactor (sigma_cda.cc:62-77)."""
+        if e_f < e_m < w:
+            return 1.0
+        if e_f > e_m > w:
+            return -1.0
+        if abs(e_m - w) < tol and e_f > e_m:
+            return -0.5
+        if abs(e_m - w) < tol and e_f < e_m:
+            return 0.5
+        return 0.0
+
+    def residues(self, omega, level, rpa_eta=1e-4):
+        """Sigma_CDA::CalcResidueContribution + tail (sigma_cda.cc:79-141)."""
+        homo = self.homo - self.rpamin
+        fermi = 0.5 * (self.e[homo + 1] + self.e[homo])
+        Imx = self.M[level + self.qpmin - self.rpamin]
+        s, tail = 0.0, 0.0
+        for i in range(len(self.e)):
+            d = self.e[i] - omega
+            fac = self.prefactor(fermi, self.e[i], omega)
+            if abs(fac) > 1e-10:
+                eps = rpa_epsilon(self.M, self.e, self.homo, self.rpamin, self.rpamax, abs(d) + 1j * rpa_eta, "complex")
+                x = np.linalg.solve(eps, Imx[i]) - Imx[i]
+                s += fac * float(x @ Imx[i])
+            if abs(d) > 1e-10:
+                from scipy.special import erfc
+                ef = 0.5 * np.copysign(1.0, d) * np.exp((self.alpha * d) ** 2) * erfc(abs(self.alpha * d))
+                tail += float(Imx[i] @ self.k0 @ Imx[i]) * ef
+        return s + tail
+
+    def diag(self, omega, level):
+        return self.residues(omega, level) + self.sigma_gq(omega, level)

@@ -367,3 +367,60 @@ def test_adjoint_matches_finite_differences(ctx, full):
               - omegas(_perturb_sym(M, P, v, c, vmin - rpamin, homo + 1 - rpamin, -h), Hqp)) / (2 * h)
         an = d_m[:, P, v, c]
         assert np.abs(fd - an).max() < 1e-5 * max(1e-3, np.abs(fd).max()), (P, v, c, fd, an)
+
+
+def test_autograd_function_matches_finite_differences(ctx):
+    import torch
+    from votca_amd.autograd import BSEProblem, excitation_energies
+    M, eps, Hqp, homo, rpamin, vmin, cmax = random_bse_case(60, vt=4, ct=18, P=24, scale=0.05)
+    vt, ct = homo - vmin + 1, cmax - homo
+    t = dev_mmn(ctx, M)
+    prob = BSEProblem(t, eps, homo, rpamin, vmin, cmax, nroots=2, full=True, tol=1e-10)
+    e = torch.tensor(np.diag(Hqp).copy(), dtype=torch.float64, requires_grad=True)
+    m = torch.tensor(np.transpose(M[0:vt, vt:vt + ct, :], (2, 0, 1)).copy(), dtype=torch.float64,
+                     requires_grad=True)
+    w = torch.tensor([0.7, -0.3], dtype=torch.float64)
+    loss = (excitation_energies(e, m, prob) * w).sum()
+    loss.backward()
+    h = 1e-5
+
+    def L(ev, mv):
+        with torch.no_grad():
+            return float((excitation_energies(ev, mv, prob) * w).sum())
+
+    for lvl in (0, vt + 1):
+        ep, em = e.detach().clone(), e.detach().clone()
+        ep[lvl] += h
+        em[lvl] -= h
+        fd = (L(ep, m.detach()) - L(em, m.detach())) / (2 * h)
+        assert abs(fd - float(e.grad[lvl])) < 1e-5 * max(1.0, abs(fd))
+    for (P, v, c) in ((0, 1, 2), (5, 3, 10)):
+        mp, mm = m.detach().clone(), m.detach().clone()
+        mp[P, v, c] += h
+        mm[P, v, c] -= h
+        fd = (L(e.detach(), mp) - L(e.detach(), mm)) / (2 * h)
+        assert abs(fd - float(m.grad[P, v, c])) < 1e-5 * max(1e-3, abs(fd))
+
+
+CDA_E = np.array([0.0468207, 0.0907801, 0.0907801, 0.104563, 0.592491, 0.663355, 0.663355, 0.768373, 1.69292,
+                  1.97724, 1.97724, 2.50877, 2.98732, 3.4418, 3.4418, 4.81084, 17.1838])
+
+
+def test_sigma_cda_fixture_gpu(ctx, ints):  # test_sigma_cda.cc:41-113
+    t = ch4_tensor(ctx, ints, "sigma_cda/MOs.mm")
+    M = t.get_tensor()
+    rpa = px.RPA(t)
+    rpa.configure(4, 0, 16)
+    rpa.setRPAInputEnergies(CDA_E)
+    cda = px.Sigma_CDA(t, rpa).configure(4, 0, 16, 0, 16, eta=1e-3, alpha=1e-3, quadrature_scheme="legendre",
+                                         order=100)
+    x = cda.CalcExchangeMatrix()
+    assert mmio.isapprox(x, mmio.read_matrix(os.path.join(G, "sigma_cda/x_ref.mm")), 1e-4)
+    cda.PrepareScreening()
+    c = cda.CalcCorrelationDiag(CDA_E)
+    assert mmio.isapprox(c, np.diag(mmio.read_matrix(os.path.join(G, "sigma_cda/c_ref.mm"))), 1e-5)
+    from votca_amd import quadrature as Q
+    q = Q.create("legendre", 100)
+    ora = X.SigmaCDAOracle(M, CDA_E, 4, 0, 16, 0, 1e-3, 1e-3, q.points(), q.weights(), q.use_symmetry())
+    for l, w in ((3, -0.2), (7, 0.9), (12, 2.5)):
+        assert abs(cda.diag_batch([l], [w])[0] - ora.diag(w, l)) < 1e-9

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <tuple>
 #include <cmath>
 #include <memory>
 #include <numeric>
@@ -419,6 +420,34 @@ extern "C" int xtpgpu_mmn_fill_synthetic(xtpgpu_mmn mmn, uint64_t seed, double s
       long m = r / nt + mmin, q = r % nt + nmin;
       long a = m < q ? m : q, b = m < q ? q : m;
       D[i] = synth_normal(seed, (uint64_t)p, (uint64_t)a, (uint64_t)b) * s[p];
+    });
+    XG_HIP(hipStreamSynchronize(ctx->stream));
+    mmn->version++;
+  });
+}
+
+extern "C" int xtpgpu_mmn_set_block(xtpgpu_mmn mmn, int64_t vmin, int64_t vtotal, int64_t cmin, int64_t ctotal,
+                                    const double* m, int symmetric) {
+  return guarded([&] {
+    XG_REQUIRE(mmn && m, XTPGPU_ERR_ARG, "xtpgpu_mmn_set_block: null argument");
+    XG_REQUIRE(vmin >= 0 && cmin >= 0 && vtotal > 0 && ctotal > 0 && vmin + vtotal <= mmn->mtotal &&
+                   cmin + ctotal <= mmn->ntotal &&
+                   (!symmetric || (cmin + ctotal <= mmn->mtotal && vmin + vtotal <= mmn->ntotal)),
+               XTPGPU_ERR_SHAPE, "xtpgpu_mmn_set_block: block outside the tensor");
+    xtpgpu_ctx ctx = mmn->ctx;
+    XG_HIP(hipSetDevice(ctx->device));
+    const long P = mmn->aux, n = (long)P * vtotal * ctotal;
+    DevBuf b(n * 8);
+    XG_HIP(hipMemcpyAsync(b.p, m, n * 8, hipMemcpyHostToDevice, ctx->stream));
+    const double* s = b.as<double>();
+    double* D = mmn->d();
+    const long nt = mmn->ntotal, ps = (long)mmn->pstride(), vt = vtotal, ct = ctotal, v0 = vmin, c0 = cmin;
+    const bool sym = symmetric != 0;
+    launch_map(ctx->stream, (size_t)n, [=] __device__(size_t i) {
+      long p = (long)(i / (vt * ct)), r = (long)(i % (vt * ct));
+      long v = r / ct, c = r % ct;
+      D[p * ps + (v0 + v) * nt + c0 + c] = s[i];
+      if (sym) D[p * ps + (c0 + c) * nt + v0 + v] = s[i];
     });
     XG_HIP(hipStreamSynchronize(ctx->stream));
     mmn->version++;

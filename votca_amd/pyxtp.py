@@ -76,6 +76,7 @@ SIGNATURES = {
     "xtpgpu_mmn_multiply_right": (C.c_int, [_vp, _dp, _i64]),
     "xtpgpu_mmn_fill_ao2mo": (C.c_int, [_vp, _dp, _i64, _dp, _i64, _i64]),
     "xtpgpu_mmn_fill_synthetic": (C.c_int, [_vp, C.c_uint64, C.c_double, C.c_double]),
+    "xtpgpu_mmn_set_block": (C.c_int, [_vp, _i64, _i64, _i64, _i64, _dp, C.c_int]),
     "xtpgpu_mmn_sizes": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)]),
     "xtpgpu_rpa_epsilon": (C.c_int, [_vp, _vp, _dp, _i64, _i64, _i64, C.c_int, _dp, _dp, C.c_int,
                                      C.c_double, C.c_int, _dp]),
@@ -94,6 +95,10 @@ SIGNATURES = {
                                              C.POINTER(DavidsonInfo)]),
     "xtpgpu_bse_adjoint": (C.c_int, [_vp, _vp, _i64, _dp, _dp, _i64, _dp, _dp]),
     "xtpgpu_bse_adjoint_device": (C.c_int, [_vp, _vp, _i64, _dp, _dp, _i64, _dp, _vp]),
+    "xtpgpu_sigma_cda_create": (C.c_int, [_vp, _vp, _dp, _i64, _i64, _i64, _i64, _i64, C.c_double, C.c_double,
+                                          _dp, _dp, _i64, C.c_int, C.POINTER(_vp)]),
+    "xtpgpu_sigma_cda_destroy": (C.c_int, [_vp]),
+    "xtpgpu_sigma_cda_diag": (C.c_int, [_vp, _i64, C.POINTER(_i64), _dp, _dp, _dp]),
     "xtpgpu_bse_setup_screening": (C.c_int, [_vp, _vp, _dp, _i64, _i64, _i64, C.c_double, _dp]),
     "xtpgpu_sigma_x": (C.c_int, [_vp, _vp, _i64, _i64, _i64, _i64, _dp]),
     "xtpgpu_ppm_prepare": (C.c_int, [_vp, _vp, _dp, _i64, _i64, _i64, _dp, _dp]),
@@ -284,6 +289,13 @@ class TCMatrix_gwbse:
         V^-1/2 (AOCoulomb::Pseudo_InvSqrt_GWBSE, computed by the caller)."""
         self.Fill3cMO(ao3c, C_mo)
         self.MultiplyRightWithAuxMatrix(inv_sqrt)
+
+    def set_block(self, vmin: int, cmin: int, m: np.ndarray, symmetric: bool = True):
+        """m[P, v, c] -> Mmn_[vmin+v](cmin+c, P) (and the (c, v) partner when symmetric)."""
+        mm = np.ascontiguousarray(m, dtype=np.float64)
+        P, vt, ct = mm.shape
+        assert P == self.aux
+        _check(lib().xtpgpu_mmn_set_block(self.h, vmin, vt, cmin, ct, _dptr(mm), 1 if symmetric else 0))
 
     def fill_synthetic(self, seed: int, sigma0: float = 0.3, decay: float = 3.0):
         _check(lib().xtpgpu_mmn_fill_synthetic(self.h, C.c_uint64(seed), sigma0, decay))
@@ -667,3 +679,58 @@ def bse_adjoint(A: BSE_OPERATOR, X, B: BSE_OPERATOR | None = None, Y=None, want_
                                     _dptr(Yf) if Yf is not None else None, Xf.shape[0], _dptr(d_eps),
                                     _dptr(d_m) if want_m else None))
     return d_eps, d_m
+
+
+class Sigma_CDA:
+    """Sigma_CDA (sigma_cda.h) on the device; Sigma_x from the same tensor."""
+
+    def __init__(self, Mmn: TCMatrix_gwbse, rpa: RPA):
+        self.Mmn, self.rpa, self.h = Mmn, rpa, None
+
+    def configure(self, homo, qpmin, qpmax, rpamin, rpamax, eta=1e-3, alpha=1e-3, quadrature_scheme="legendre",
+                  order=12):
+        from . import quadrature
+        self.homo, self.qpmin, self.qpmax, self.rpamin, self.rpamax = homo, qpmin, qpmax, rpamin, rpamax
+        self.eta, self.alpha, self.qptotal = eta, alpha, qpmax - qpmin + 1
+        self.quad = quadrature.create(quadrature_scheme, order)
+        return self
+
+    def CalcExchangeMatrix(self):
+        out = np.zeros((self.qptotal, self.qptotal), order="F")
+        _check(lib().xtpgpu_sigma_x(self.Mmn.ctx.h, self.Mmn.h, self.homo, self.rpamin, self.qpmin, self.qpmax,
+                                    _dptr(out)))
+        return out
+
+    def PrepareScreening(self):
+        self.close()
+        e = np.ascontiguousarray(self.rpa.getRPAInputEnergies())
+        x = np.ascontiguousarray(self.quad.points())
+        w = np.ascontiguousarray(self.quad.weights())
+        h = _vp()
+        _check(lib().xtpgpu_sigma_cda_create(self.Mmn.ctx.h, self.Mmn.h, _dptr(e), self.homo, self.rpamin,
+                                             self.rpamax, self.qpmin, self.qpmax, self.eta, self.alpha, _dptr(x),
+                                             _dptr(w), len(x), 1 if self.quad.use_symmetry() else 0, C.byref(h)))
+        self.h = h
+
+    def diag_batch(self, levels, omegas, derivative=False):
+        lv = np.ascontiguousarray(levels, dtype=np.int64)
+        om = np.ascontiguousarray(omegas, dtype=np.float64)
+        s = np.zeros(len(om))
+        d = np.zeros(len(om)) if derivative else None
+        _check(lib().xtpgpu_sigma_cda_diag(self.h, len(om), lv.ctypes.data_as(C.POINTER(_i64)), _dptr(om), _dptr(s),
+                                           _dptr(d) if derivative else None))
+        return (s, d) if derivative else s
+
+    def CalcCorrelationDiag(self, frequencies):
+        return self.diag_batch(np.arange(self.qptotal), frequencies)
+
+    def close(self):
+        if self.h:
+            lib().xtpgpu_sigma_cda_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
