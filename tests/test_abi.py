@@ -33,3 +33,27 @@ def test_binding_covers_header():
 def test_binding_loads():
     L = pyxtp.lib()
     assert L.xtpgpu_last_error() is not None
+
+
+def test_cpp_header_compiles_and_links(tmp_path):
+    """include/xtpgpu.hpp (the C++ surface shaped like the reference classes) compiles and links
+    against the library; nothing is executed."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        import pytest
+        pytest.skip("no g++")
+    src = tmp_path / "use.cc"
+    src.write_text(
+        '#include "xtpgpu.hpp"\n'
+        "int main(int argc, char**) {\n"
+        "  if (argc > 5) { xtpgpu::Context c(0); xtpgpu::TCMatrix t(c, 4, 0, 3, 0, 3);\n"
+        "    xtpgpu::BSEOperatorOptions o{1, 0, 0, 0, 3};\n"
+        "    xtpgpu::SingletOperator_TDA H(c, t, nullptr, nullptr, 4, o); return (int)H.rows(); }\n"
+        "  return 0; }\n")
+    out = tmp_path / "use"
+    r = subprocess.run([gxx, "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(out),
+                        "-L", os.path.dirname(pyxtp.LIB_PATH), "-lxtpgpu",
+                        "-Wl,-rpath," + os.path.dirname(pyxtp.LIB_PATH)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
