@@ -86,6 +86,99 @@ def cpu_baseline(o, v, P, seed, n_iter_fp64, n_calls, budget_s=20.0):
     }
 
 
+def run_segments(args):
+    """Batched uneven segments: n_occ ~ U[10,60], n_virt = n_occ*U[5,8], n_aux = 4(n_occ+n_virt).
+    eps(i w) at 12 frequencies for all segments in one persistent launch (hipGraph replay) against a
+    serial per-segment loop, then singlet BSE-TDA (8 roots) per segment; segments/s and p50/p99."""
+    from votca_amd import pyxtp as px
+    from votca_amd import quadrature
+    rng = np.random.default_rng(args.seed)
+    nseg = args.nseg
+    ctx = px.Context(0)
+    freqs = quadrature.create("modified_legendre", 12).points()
+    segs = []
+    for s in range(nseg):
+        o = int(rng.integers(10, 61))
+        v = int(round(o * rng.uniform(5.0, 8.0)))
+        P = 4 * (o + v)
+        segs.append((o, v, P))
+    log(f"[segments] {nseg} segments, o in [{min(x[0] for x in segs)},{max(x[0] for x in segs)}], "
+        f"P in [{min(x[2] for x in segs)},{max(x[2] for x in segs)}]")
+    rpas = []
+    for s, (o, v, P) in enumerate(segs):
+        t = px.TCMatrix_gwbse(ctx).Initialize(P, 0, o - 1, 0, o + v - 1)   # occupied slices: all eps needs
+        t.fill_synthetic(args.seed * 100003 + s, args.sigma0, 3.0)
+        r = px.RPA(t)
+        r.configure(o - 1, 0, o + v - 1)
+        r.setRPAInputEnergies(synthetic_energies(args.seed * 100003 + s, o, v))
+        rpas.append(r)
+    prec = args.mixed if args.segments_precision == "mixed" else "fp64"
+    batch = px.EpsBatch(rpas, freqs, prec)
+    # autotune the persistent grid (workgroups per CU) on this batch
+    best = None
+    for mult in (1, 2, 4, 8):
+        batch.set_grid(256 * mult)
+        batch.run(True)
+        t0 = time.perf_counter()
+        batch.run(True)
+        dt = time.perf_counter() - t0
+        log(f"[segments] autotune grid={256*mult}: {dt*1e3:.1f} ms")
+        if best is None or dt < best[1]:
+            best = (mult, dt)
+    batch.set_grid(256 * best[0])
+    batch.run(True)
+    ts = []
+    for _ in range(max(1, args.steps)):
+        t0 = time.perf_counter()
+        batch.run(True)
+        ts.append(time.perf_counter() - t0)
+    t_batch = float(np.median(ts))
+    # serial per-segment loop with the single-segment entry point
+    lat = []
+    bufs = {}
+    t_all = time.perf_counter()
+    for r in rpas:
+        P = r.Mmn.aux
+        if P not in bufs:
+            bufs[P] = px.DeviceBuffer(ctx, P * P * 12 * 8)
+        t0 = time.perf_counter()
+        px._check(px.lib().xtpgpu_rpa_epsilon_device(ctx.h, r.Mmn.h, px._dptr(r.energies), r.homo, 0, r.rpamax,
+                                                      px.EPS_IMAG, px._dptr(np.ascontiguousarray(freqs)), None,
+                                                      12, 1e-4, px._PREC[prec], bufs[P].ptr))
+        lat.append(time.perf_counter() - t0)
+    t_serial = time.perf_counter() - t_all
+    F = sum(12.0 * o * v * P * (P + 1) for (o, v, P) in segs)
+    res = {"eps_batch_s": t_batch, "eps_serial_s": t_serial, "eps_batch_segments_per_s": nseg / t_batch,
+           "eps_serial_segments_per_s": nseg / t_serial, "eps_batch_tflops": F / t_batch / 1e12,
+           "eps_serial_latency_ms_p50": float(np.percentile(lat, 50) * 1e3),
+           "eps_serial_latency_ms_p99": float(np.percentile(lat, 99) * 1e3), "autotuned_grid": 256 * best[0],
+           "precision": prec}
+    log(f"[segments] eps batch {t_batch*1e3:.1f} ms ({nseg/t_batch:.1f} seg/s) vs serial {t_serial*1e3:.1f} ms")
+    del batch, rpas, bufs
+    # BSE-TDA, 8 roots per segment (full tensors, one segment at a time)
+    lat_bse = []
+    nb = min(nseg, args.bse_segments)
+    for s, (o, v, P) in enumerate(segs[:nb]):
+        t0 = time.perf_counter()
+        t = px.TCMatrix_gwbse(ctx).Initialize(P, 0, o + v - 1, 0, o + v - 1)
+        t.fill_synthetic(args.seed * 100003 + s, args.sigma0, 3.0)
+        e = synthetic_energies(args.seed * 100003 + s, o, v)
+        ei = px.setup_screening(t, e, o - 1, 0, o + v - 1)
+        op = px.SingletOperator_TDA(ei, t, np.diag(e)).configure(px.BSEOperator_Options(o - 1, 0, 0, 0, o + v - 1))
+        ds = px.DavidsonSolver()
+        ds.set_tolerance(1e-5)
+        ds.set_iter_max(100)
+        ds.set_max_search_space(80)
+        ds.solve(op, 8, want_vectors=False)
+        lat_bse.append(time.perf_counter() - t0)
+        op.close()
+        t.close()
+    res.update({"bse_segments": nb, "bse_latency_ms_p50": float(np.percentile(lat_bse, 50) * 1e3),
+                "bse_latency_ms_p99": float(np.percentile(lat_bse, 99) * 1e3),
+                "bse_segments_per_s": nb / float(np.sum(lat_bse))})
+    print(json.dumps({"workload": "batched uneven segments (configs[3])", "n_segments": nseg, **res}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,12 +197,19 @@ def main():
                          "on which the reference Davidson does not converge (exchange ~100x physical)")
     ap.add_argument("--iter-max", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="single", choices=["single", "segments"])
+    ap.add_argument("--nseg", type=int, default=256)
+    ap.add_argument("--bse-segments", type=int, default=32)
+    ap.add_argument("--segments-precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--skip-fp64-solve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
 
+    if args.workload == "segments":
+        run_segments(args)
+        return
     from votca_amd import pyxtp as px
     from votca_amd import quadrature
 

@@ -248,6 +248,22 @@ int xtpgpu_davidson_solve_full(xtpgpu_bseop A, xtpgpu_bseop B, int64_t neigen,
                                const xtpgpu_davidson_opts* opts, double* evals, double* X,
                                double* Y, int64_t ldv, xtpgpu_davidson_info* info);
 
+/* ---------------------------------------------------------------- batched segments */
+/* eps(i w) for many segments of uneven size (the per-segment calculate_epsilon_i of
+ * rpa.cc:75-140 that the reference runs job by job, parallelxjobcalc.cc:52-111) as one
+ * persistent launch over (segment, frequency, lower-triangle tile) work items. create packs the
+ * segments' transition blocks and weights; run launches (optionally as a captured hipGraph replay);
+ * result copies segment seg's eps (P x P x n_omega, column-major) to the host. */
+typedef struct xtpgpu_epsbatch_s* xtpgpu_epsbatch;
+int xtpgpu_epsbatch_create(xtpgpu_ctx ctx, int nseg, const xtpgpu_mmn* tensors,
+                           const double* const* energies, const int64_t* homo, const int64_t* rpamin,
+                           const int64_t* rpamax, int n_omega, const double* omegas, int precision,
+                           int grid, xtpgpu_epsbatch* out);
+int xtpgpu_epsbatch_run(xtpgpu_epsbatch b, int use_graph);
+int xtpgpu_epsbatch_set_grid(xtpgpu_epsbatch b, int grid);
+int xtpgpu_epsbatch_result(xtpgpu_epsbatch b, int seg, double* out);
+int xtpgpu_epsbatch_destroy(xtpgpu_epsbatch b);
+
 /* ---------------------------------------------------------------- adjoint */
 /* Hellmann-Feynman derivatives of converged BSE roots at fixed screening, the contraction
  * pattern of BSE::ExpectationValue_Operator (bse.cc:490-606): dOmega = X^T dA X + Y^T dA Y +

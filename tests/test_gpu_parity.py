@@ -424,3 +424,27 @@ def test_sigma_cda_fixture_gpu(ctx, ints):  # test_sigma_cda.cc:41-113
     ora = X.SigmaCDAOracle(M, CDA_E, 4, 0, 16, 0, 1e-3, 1e-3, q.points(), q.weights(), q.use_symmetry())
     for l, w in ((3, -0.2), (7, 0.9), (12, 2.5)):
         assert abs(cda.diag_batch([l], [w])[0] - ora.diag(w, l)) < 1e-9
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("bf16x3", 2e-5)])
+def test_eps_batch_matches_per_segment(ctx, prec, tol):
+    rng = np.random.default_rng(70)
+    rpas, refs = [], []
+    freqs = [0.0, 0.4, 2.0]
+    for s_ in range(5):
+        o = int(rng.integers(3, 9))
+        v = int(o * rng.uniform(5, 8))
+        P = 4 * (o + v) // 3
+        M = rng.normal(scale=0.2, size=(o + v, o + v, P))
+        e = np.concatenate([np.sort(rng.uniform(-1, -0.2, o)), np.sort(rng.uniform(0, 1.5, v))])
+        t = dev_mmn(ctx, M)
+        r = px.RPA(t)
+        r.configure(o - 1, 0, o + v - 1)
+        r.setRPAInputEnergies(e)
+        rpas.append(r)
+        refs.append(np.stack([X.rpa_epsilon(M, e, o - 1, 0, o + v - 1, f, "imag") for f in freqs], axis=2))
+    b = px.EpsBatch(rpas, freqs, prec)
+    for use_graph in (False, True, True):
+        b.run(use_graph)
+        for s_ in range(5):
+            assert relerr(b.result(s_), refs[s_]) < tol

@@ -79,9 +79,9 @@ __device__ __forceinline__ float bf16_bits_to_f(unsigned short h) {
 // Block tile BM x BN x 16, 256 threads = 2x2 waves, wave tile (BM/2) x (BN/2).
 // VEC: operands are K-contiguous with ld, base and K bounds multiples of 2 (16-byte loads).
 template <int BM, int BN, bool AK, bool BK_, bool HAS_S, bool VEC, class Epi>
-__global__ __launch_bounds__(256) void gemm_f64_kernel(GemmDims d, Opnd<double> A, Opnd<double> B,
-                                                       const double* __restrict__ s, long s_bs,
-                                                       Epi epi) {
+__device__ __forceinline__ void gemm_f64_tile(const GemmDims& d, const Opnd<double>& A, const Opnd<double>& B,
+                                              const double* __restrict__ s, long s_bs, const Epi& epi, int tm,
+                                              int tn, int b, int ks) {
   constexpr int BK = 16;
   constexpr int PA = BM + 1;  // padded LDS row (doubles)
   constexpr int PB = BN + 1;
@@ -98,11 +98,7 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmDims d, Opnd<double> 
   const int wm0 = (wave >> 1) * WM;
   const int wn0 = (wave & 1) * WN;
 
-  int tm, tn;
-  tile_of_block(d, tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
-  const int b = blockIdx.z;
-  const int ks = blockIdx.y;
   const int kbeg = ks * d.kchunk;
   const int kend = min(d.K, kbeg + d.kchunk);
 
@@ -240,6 +236,15 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmDims d, Opnd<double> 
         int gn = n0 + wn0 + j * 16 + (lane & 15);
         if (gm < d.M && gn < d.N) epi(b, ks, gm, gn, acc[i][j][r]);
       }
+  __syncthreads();  // LDS is reused by the next tile of a persistent caller
+}
+
+template <int BM, int BN, bool AK, bool BK_, bool HAS_S, bool VEC, class Epi>
+__global__ __launch_bounds__(256) void gemm_f64_kernel(GemmDims d, Opnd<double> A, Opnd<double> B,
+                                                       const double* __restrict__ s, long s_bs, Epi epi) {
+  int tm, tn;
+  tile_of_block(d, tm, tn);
+  gemm_f64_tile<BM, BN, AK, BK_, HAS_S, VEC, Epi>(d, A, B, s, s_bs, epi, tm, tn, blockIdx.z, blockIdx.y);
 }
 
 // ------------------------------------------------------------------ BF16 kernel
@@ -249,10 +254,9 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmDims d, Opnd<double> 
 // Block tile BM x BN x 32, 256 threads = 2x2 waves.
 // VEC: operands are K-contiguous with ld, base and K bounds multiples of 8 (16-byte loads).
 template <int BM, int BN, bool AK, bool BK_, bool SPLIT, bool HAS_S, bool VEC, class Epi>
-__global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmDims d, Opnd<unsigned short> A,
-                                                        Opnd<unsigned short> B,
-                                                        const float* __restrict__ s, long s_bs,
-                                                        Epi epi) {
+__device__ __forceinline__ void gemm_bf16_tile(const GemmDims& d, const Opnd<unsigned short>& A,
+                                               const Opnd<unsigned short>& B, const float* __restrict__ s,
+                                               long s_bs, const Epi& epi, int tm, int tn, int b, int ks) {
   constexpr int BK = 32;
   constexpr int PK = BK + 8;  // padded LDS row (bf16 elements), rows stay 16-byte aligned
   constexpr int WM = BM / 2, WN = BN / 2;
@@ -269,11 +273,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmDims d, Opnd<unsigne
   const int wm0 = (wave >> 1) * WM;
   const int wn0 = (wave & 1) * WN;
 
-  int tm, tn;
-  tile_of_block(d, tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
-  const int b = blockIdx.z;
-  const int ks = blockIdx.y;
   const int kbeg = ks * d.kchunk;
   const int kend = min(d.K, kbeg + d.kchunk);
 
@@ -451,6 +451,15 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmDims d, Opnd<unsigne
         int gn = n0 + wn0 + j * 16 + (lane & 15);
         if (gm < d.M && gn < d.N) epi(b, ks, gm, gn, acc[i][j][r]);
       }
+  __syncthreads();  // LDS is reused by the next tile of a persistent caller
+}
+
+template <int BM, int BN, bool AK, bool BK_, bool SPLIT, bool HAS_S, bool VEC, class Epi>
+__global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmDims d, Opnd<unsigned short> A, Opnd<unsigned short> B,
+                                                        const float* __restrict__ s, long s_bs, Epi epi) {
+  int tm, tn;
+  tile_of_block(d, tm, tn);
+  gemm_bf16_tile<BM, BN, AK, BK_, SPLIT, HAS_S, VEC, Epi>(d, A, B, s, s_bs, epi, tm, tn, blockIdx.z, blockIdx.y);
 }
 
 // ------------------------------------------------------------------ host launch helpers

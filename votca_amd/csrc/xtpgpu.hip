@@ -118,6 +118,13 @@ template <bool AK, bool BK_, class Epi>
 void gemm64(hipStream_t st, int M, int N, int K, int batch, int ksplit, Opnd<double> A, Opnd<double> B,
             const double* s, long s_bs, bool lower, Epi epi) {
   if (M <= 0 || N <= 0 || batch <= 0) return;
+  // 128x128 tiles halve the operand traffic per FLOP; used when the grid still fills the chip
+  const long big_tiles = (long)((M + 127) / 128) * ((N + 127) / 128) * batch * std::max(ksplit, 1);
+  if (ksplit <= 1 && M >= 256 && N >= 256 && big_tiles >= 512) {
+    GemmDims d = make_dims<128, 128>(M, N, std::max(K, 0), batch, ksplit, 16, lower);
+    XG_HIP((launch_gemm_f64<128, 128, AK, BK_>(st, d, A, B, s, s_bs, epi)));
+    return;
+  }
   GemmDims d = make_dims<64, 64>(M, N, std::max(K, 0), batch, ksplit, 16, lower);
   XG_HIP((launch_gemm_f64<64, 64, AK, BK_>(st, d, A, B, s, s_bs, epi)));
 }
@@ -1033,3 +1040,4 @@ extern "C" int xtpgpu_bse_matmul(xtpgpu_bseop op, int precision, const double* X
 #include "davidson_ham.inc"
 #include "adjoint.inc"
 #include "gw.inc"
+#include "batched.inc"

@@ -99,6 +99,13 @@ SIGNATURES = {
                                           _dp, _dp, _i64, C.c_int, C.POINTER(_vp)]),
     "xtpgpu_sigma_cda_destroy": (C.c_int, [_vp]),
     "xtpgpu_sigma_cda_diag": (C.c_int, [_vp, _i64, C.POINTER(_i64), _dp, _dp, _dp]),
+    "xtpgpu_epsbatch_create": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(_dp), C.POINTER(_i64),
+                                         C.POINTER(_i64), C.POINTER(_i64), C.c_int, _dp, C.c_int, C.c_int,
+                                         C.POINTER(_vp)]),
+    "xtpgpu_epsbatch_run": (C.c_int, [_vp, C.c_int]),
+    "xtpgpu_epsbatch_set_grid": (C.c_int, [_vp, C.c_int]),
+    "xtpgpu_epsbatch_result": (C.c_int, [_vp, C.c_int, _dp]),
+    "xtpgpu_epsbatch_destroy": (C.c_int, [_vp]),
     "xtpgpu_bse_setup_screening": (C.c_int, [_vp, _vp, _dp, _i64, _i64, _i64, C.c_double, _dp]),
     "xtpgpu_sigma_x": (C.c_int, [_vp, _vp, _i64, _i64, _i64, _i64, _dp]),
     "xtpgpu_ppm_prepare": (C.c_int, [_vp, _vp, _dp, _i64, _i64, _i64, _dp, _dp]),
@@ -727,6 +734,50 @@ class Sigma_CDA:
     def close(self):
         if self.h:
             lib().xtpgpu_sigma_cda_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class EpsBatch:
+    """eps(i w) for many segments in one persistent launch (xtpgpu_epsbatch_*)."""
+
+    def __init__(self, rpas, omegas, precision="fp64", grid=0):
+        self.rpas = list(rpas)
+        n = len(self.rpas)
+        ctx = self.rpas[0].ctx
+        self.ctx = ctx
+        tens = (_vp * n)(*[r.Mmn.h for r in self.rpas])
+        self._e = [np.ascontiguousarray(r.energies) for r in self.rpas]
+        eptr = (_dp * n)(*[_dptr(e) for e in self._e])
+        homo = (_i64 * n)(*[r.homo for r in self.rpas])
+        rmin = (_i64 * n)(*[r.rpamin for r in self.rpas])
+        rmax = (_i64 * n)(*[r.rpamax for r in self.rpas])
+        self.om = np.ascontiguousarray(omegas, dtype=np.float64)
+        h = _vp()
+        _check(lib().xtpgpu_epsbatch_create(ctx.h, n, tens, eptr, homo, rmin, rmax, len(self.om), _dptr(self.om),
+                                            _PREC[precision], grid, C.byref(h)))
+        self.h = h
+
+    def run(self, use_graph=True):
+        _check(lib().xtpgpu_epsbatch_run(self.h, 1 if use_graph else 0))
+
+    def set_grid(self, grid):
+        _check(lib().xtpgpu_epsbatch_set_grid(self.h, int(grid)))
+
+    def result(self, seg):
+        P = self.rpas[seg].Mmn.aux
+        out = np.zeros((P, P, len(self.om)), order="F")
+        _check(lib().xtpgpu_epsbatch_result(self.h, seg, _dptr(out)))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().xtpgpu_epsbatch_destroy(self.h)
             self.h = None
 
     def __del__(self):
