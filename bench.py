@@ -176,7 +176,44 @@ def run_segments(args):
     res.update({"bse_segments": nb, "bse_latency_ms_p50": float(np.percentile(lat_bse, 50) * 1e3),
                 "bse_latency_ms_p99": float(np.percentile(lat_bse, 99) * 1e3),
                 "bse_segments_per_s": nb / float(np.sum(lat_bse))})
-    print(json.dumps({"workload": "batched uneven segments (configs[3])", "n_segments": nseg, **res}))
+    print(json.dumps({"workload": f"{nseg} uneven synthetic segments, n_occ U[10,60], n_virt n_occ*U[5,8], n_aux 4(n_occ+n_virt)", "n_segments": nseg, **res}))
+
+
+def run_matvec(args):
+    """H X for the singlet BSE-TDA operator of the single-segment workload (no W rotation: eps^-1 = 1),
+    `--cols` columns, `--steps` repetitions per precision: the profiling target for the Hd stages."""
+    from votca_amd import pyxtp as px
+    o, v, P = args.nocc, args.nvirt, args.naux
+    n = o + v
+    ctx = px.Context(0)
+    Mmn = px.TCMatrix_gwbse(ctx).Initialize(P, 0, n - 1, 0, n - 1)
+    Mmn.fill_synthetic(args.seed, args.sigma0, 3.0)
+    e = synthetic_energies(args.seed, o, v)
+    op = px.SingletOperator_TDA(np.ones(P), Mmn, np.diag(e), precisions=("fp64", args.mixed)).configure(
+        px.BSEOperator_Options(o - 1, 0, 0, 0, n - 1))
+    N, k = op.size, args.cols
+    X = px.DeviceBuffer(ctx, N * k * 8)
+    X.upload(np.random.default_rng(args.seed).normal(size=(N, k)))
+    Y = px.DeviceBuffer(ctx, N * k * 8)
+    ctx.set_profiling(True)
+    out = {"workload": f"BSE-TDA H X, n_occ={o} n_virt={v} n_aux={P}, {k} columns", "steps": args.steps}
+    for prec in ("fp64", args.mixed):
+        op.matmul_device(X, k, Y, prec)
+        ctx.synchronize()
+        ctx.reset_timers()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            op.matmul_device(X, k, Y, prec)
+        ctx.synchronize()
+        wall = (time.perf_counter() - t0) / args.steps
+        tm = {name: ctx.timer(name) for name in ("hd_stage1", "hd_stage2", "hx")}
+        out[prec] = {"wall_ms": wall * 1e3,
+                     **{f"{name}_tflops": t["flops"] / max(t["ms"] / 1e3, 1e-30) / 1e12 for name, t in tm.items()},
+                     **{f"{name}_ms": t["ms"] / max(1, args.steps) for name, t in tm.items()}}
+        log(f"[matvec] {prec}: {wall*1e3:.2f} ms per H X ({k} cols); " +
+            ", ".join(f"{nm} {t['ms']/args.steps:.2f} ms {t['flops']/max(t['ms']/1e3,1e-30)/1e12:.1f} TF"
+                      for nm, t in tm.items()))
+    print(json.dumps(out))
 
 
 def main():
@@ -197,7 +234,8 @@ def main():
                          "on which the reference Davidson does not converge (exchange ~100x physical)")
     ap.add_argument("--iter-max", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="single", choices=["single", "segments"])
+    ap.add_argument("--workload", default="single", choices=["single", "segments", "matvec"])
+    ap.add_argument("--cols", type=int, default=30)
     ap.add_argument("--nseg", type=int, default=256)
     ap.add_argument("--bse-segments", type=int, default=32)
     ap.add_argument("--segments-precision", default="fp64", choices=["fp64", "mixed"])
@@ -209,6 +247,9 @@ def main():
 
     if args.workload == "segments":
         run_segments(args)
+        return
+    if args.workload == "matvec":
+        run_matvec(args)
         return
     from votca_amd import pyxtp as px
     from votca_amd import quadrature
@@ -333,7 +374,7 @@ def main():
         "ms_per_step": (t_mixed + eps_res[args.mixed]["wall_s"]) * 1e3,
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3/fp32+fp64",
         "data": "synthetic",
-        "config": {"workload": "synthetic single segment (configs[1])", "n_occ": o, "n_virt": v, "n_aux": P,
+        "config": {"workload": f"synthetic single segment n_occ={o} n_virt={v} n_aux={P}", "n_occ": o, "n_virt": v, "n_aux": P,
                    "sigma0": args.sigma0,
                    "n_freq": 12, "roots": args.roots, "tol": args.tol, "mixed": args.mixed},
         "bse_tda_solve_s": {"mixed": t_mixed, "fp64": res.get("fp64", {}).get("wall_s"),

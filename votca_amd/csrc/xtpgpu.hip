@@ -12,6 +12,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
+#include <cstring>
 #include <tuple>
 #include <cmath>
 #include <memory>
@@ -20,6 +22,7 @@
 #include <vector>
 
 #include "gemm_kernels.h"
+#include "gemm_pipe.h"
 #include "host_eigen.h"
 #include "kernels_misc.h"
 #include "xtpgpu_internal.h"
@@ -30,6 +33,8 @@ static thread_local std::string g_last_error;
 
 template <class F>
 static int guarded(F&& f) {
+  // a failure returned by an earlier call must not be reported again by this one
+  (void)hipGetLastError();
   try {
     f();
     return XTPGPU_OK;
@@ -129,10 +134,29 @@ void gemm64(hipStream_t st, int M, int N, int K, int batch, int ksplit, Opnd<dou
   XG_HIP((launch_gemm_f64<64, 64, AK, BK_>(st, d, A, B, s, s_bs, epi)));
 }
 
+// The direct-to-LDS pipelined kernel serves split, K-contiguous, unscaled products; the
+// register-staged kernel the rest. XTPGPU_BF16_KERNEL=staged forces the latter (A/B timing).
+inline bool pipe_enabled() {
+  const char* e = std::getenv("XTPGPU_BF16_KERNEL");
+  return !(e && std::strcmp(e, "staged") == 0);
+}
+
 template <bool AK, bool BK_, class Epi>
 void gemm16(hipStream_t st, int M, int N, int K, int batch, int ksplit, Opnd<unsigned short> A,
             Opnd<unsigned short> B, const float* s, long s_bs, bool split, bool lower, Epi epi) {
   if (M <= 0 || N <= 0 || batch <= 0) return;
+  if constexpr (AK && BK_) {
+    if (split && !s && !lower && pipe_ok(A, B, K) && pipe_enabled()) {
+      if ((long)M * N >= 256L * 128 * 512) {
+        GemmDims d = make_dims<256, 128>(M, N, std::max(K, 0), batch, ksplit, 32, false);
+        XG_HIP((launch_gemm_bf16x3_pipe<256, 128, 3>(st, d, A, B, epi)));
+      } else {
+        GemmDims d = make_dims<128, 128>(M, N, std::max(K, 0), batch, ksplit, 32, false);
+        XG_HIP((launch_gemm_bf16x3_pipe<128, 128, 4>(st, d, A, B, epi)));
+      }
+      return;
+    }
+  }
   GemmDims d = make_dims<128, 128>(M, N, std::max(K, 0), batch, ksplit, 32, lower);
   XG_HIP((launch_gemm_bf16<128, 128, AK, BK_>(st, d, A, B, s, s_bs, split, epi)));
 }

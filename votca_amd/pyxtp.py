@@ -413,6 +413,12 @@ class BSE_OPERATOR:
         _check(lib().xtpgpu_bse_diagonal(self.h, _dptr(out)))
         return out
 
+    def matmul_device(self, X: "DeviceBuffer", k: int, Y: "DeviceBuffer", precision="fp64"):
+        """Y = H X on device-resident column-major buffers (size x k, leading dimension size)."""
+        assert X.nbytes >= self.size * k * 8 and Y.nbytes >= self.size * k * 8
+        _check(lib().xtpgpu_bse_matmul_device(self.h, _PREC[precision], X.ptr, self.size, int(k), Y.ptr,
+                                              self.size))
+
     def matmul(self, X, precision="fp64") -> np.ndarray:
         X = _f64(X)
         if X.ndim == 1:
