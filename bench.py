@@ -1,15 +1,18 @@
 """Headline benchmark: BSE-TDA 20-root solve and RPA eps build, mixed vs FP64, on one MI355X.
 
 Workload (synthetic single segment): n_occ=100, n_virt=900, n_aux=2000, seed 0.
-  * Mmn^P_{mn} = sigma_P g(seed,P,m,n), sigma_P = 0.3 exp(-3P/2000), symmetric in (m,n), for the
-    whole 1000-level window (ia, ij and ab blocks: Hd needs all three);
+  * Mmn^P_{mn} = sigma_P g(seed,P,m,n), sigma_P = sigma0 exp(-3P/2000), symmetric in (m,n), for the
+    whole 1000-level window (ia, ij and ab blocks: Hd needs all three). sigma0 defaults to 0.01:
+    at 0.3 the exchange term alone is ~30 Ha per transition and no Davidson run (the reference
+    algorithm included) converges; at 0.01 the FP64 solve needs ~250 iterations to 1e-7 Ha;
   * occupied energies sorted uniform in [-20,-5] eV, virtual sorted uniform in [0,30] eV;
   * RPA eps(i w) at the 12 modified Gauss-Legendre frequencies;
   * W from eps_r(0) (eigendecomposition, Mmn rotated, eps^-1 = 1/lambda), Hqp = diag(e);
   * singlet BSE-TDA, Davidson for 20 roots, DPR, 'safe' update, search space 200,
-    residual-norm tolerance 1e-7 Ha.
+    residual-norm tolerance 1e-7 Ha, at most 300 iterations.
 A "step" is one pass of the hot path: the 12-frequency eps build plus the 20-root BSE-TDA solve in
-the mixed-precision form (BF16 split operands, FP32 accumulation, FP64 Rayleigh-Ritz refinement).
+the mixed-precision form (BF16 split operands, FP32 accumulation, FP64 orthogonalisation and
+Rayleigh-Ritz, FP64 residual check/refinement of the converged roots).
 The plain FP64 GPU form runs once beside it for the speed-up and the accuracy check.
 
 Prints one JSON line (the driver contract) plus diagnostics on stderr.
@@ -213,6 +216,15 @@ def run_matvec(args):
         log(f"[matvec] {prec}: {wall*1e3:.2f} ms per H X ({k} cols); " +
             ", ".join(f"{nm} {t['ms']/args.steps:.2f} ms {t['flops']/max(t['ms']/1e3,1e-30)/1e12:.1f} TF"
                       for nm, t in tm.items()))
+    if args.copy_calibration:
+        # a plain 1 GiB device-to-device copy kernel: the reference reading for the FETCH_SIZE /
+        # WRITE_SIZE counters (2^28 float32 read + 2^28 float32 written)
+        import torch
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = {"read": 1 << 30, "written": 1 << 30}
     print(json.dumps(out))
 
 
@@ -220,7 +232,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--nocc", type=int, default=100)
     ap.add_argument("--nvirt", type=int, default=900)
@@ -228,14 +240,17 @@ def main():
     ap.add_argument("--roots", type=int, default=20)
     ap.add_argument("--tol", type=float, default=1e-7)
     ap.add_argument("--mixed", default="bf16x3", choices=["bf16", "bf16x3"])
-    ap.add_argument("--mixed-tol", type=float, default=1e-4)
+    ap.add_argument("--mixed-tol", type=float, default=None,
+                    help="residual tolerance of the mixed-precision loop (default: --tol)")
     ap.add_argument("--sigma0", type=float, default=0.01,
                     help="amplitude of sigma_P = sigma0 exp(-3P/n_aux); 0.3 is the literal config value, "
                          "on which the reference Davidson does not converge (exchange ~100x physical)")
-    ap.add_argument("--iter-max", type=int, default=100)
+    ap.add_argument("--iter-max", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="single", choices=["single", "segments", "matvec"])
     ap.add_argument("--cols", type=int, default=30)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="matvec workload: also run a 1 GiB copy kernel (counter calibration)")
     ap.add_argument("--nseg", type=int, default=256)
     ap.add_argument("--bse-segments", type=int, default=32)
     ap.add_argument("--segments-precision", default="fp64", choices=["fp64", "mixed"])
@@ -281,7 +296,7 @@ def main():
     eps_res = {}
     eps_ref = None
     for prec in ("fp64", args.mixed):
-        for _ in range(args.warmup):
+        for _ in range(max(1, args.warmup)):
             eps_build(prec)
         ctx.reset_timers()
         ts = [eps_build(prec) for _ in range(max(1, args.steps))]
@@ -312,13 +327,13 @@ def main():
 
     def solve(prec):
         ds = px.DavidsonSolver()
-        ds.set_tolerance(args.tol if prec == "fp64" else args.mixed_tol)
+        ds.set_tolerance(args.tol if prec == "fp64" else (args.mixed_tol or args.tol))
         ds.set_correction("DPR")
         ds.set_size_update("safe")
         ds.set_max_search_space(10 * args.roots)
         ds.set_iter_max(args.iter_max)
         if prec != "fp64":
-            ds.set_precision(prec, refine_tolerance=args.tol, refine_max=6)
+            ds.set_precision(prec, refine_tolerance=args.tol, refine_max=4)
         ctx.reset_timers()
         t0 = time.perf_counter()
         ev, _ = ds.solve(op, args.roots, want_vectors=False)
@@ -350,13 +365,17 @@ def main():
     t_mixed = float(np.median(walls))
     max_err_mev = None
     if ev64 is not None:
-        max_err_mev = float(np.abs(evm - ev64).max() * HRT2EV * 1e3)
+        per_root = np.abs(evm - ev64) * HRT2EV * 1e3
+        max_err_mev = float(per_root.max())
+        res["mixed"]["omega_err_meV_per_root"] = [float(x) for x in per_root]
+        res["fp64"]["omega_eV"] = [float(x) for x in ev64 * HRT2EV]
         log(f"[bench] max |dOmega| mixed vs fp64: {max_err_mev:.3e} meV")
 
     tm = res["mixed"]["hd_stage1"]
     kt = tm["ms"] / 1e3 / max(1, tm["launches"])
     ach = tm["flops"] / max(tm["ms"] / 1e3, 1e-30)
-    roofline = {"bound": "mfma", "kernel": "hd_stage1 (gemm_bf16_kernel)", "achieved": ach / 1e12,
+    roofline = {"bound": "mfma", "kernel": "hd_stage1 (gemm_bf16x3_pipe_kernel<256,128,3,EpiStage1>)",
+                "achieved": ach / 1e12,
                 "peak": PEAK["bf16"] / 1e12, "unit": "TFLOP/s", "frac": ach / PEAK["bf16"], "traffic": None,
                 "avg_launch_ms": kt * 1e3, "algorithmic_flops_per_launch": tm["flops"] / max(1, tm["launches"])}
 

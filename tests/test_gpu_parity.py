@@ -104,6 +104,24 @@ def test_rpa_random_multi_frequency(ctx, prec, tol):
         assert np.array_equal(g[:, :, i], g[:, :, i].T)
 
 
+@pytest.mark.parametrize("kern", ["staged32", "staged64"])
+def test_rpa_bf16x3_kernel_variants(ctx, kern, monkeypatch):
+    """eps(i w) through both register-staged BF16 kernels (K step 32 and 64; the frequency
+    weight is applied on the B-operand load) against the oracle."""
+    rng = np.random.default_rng(3)
+    o, v, P = 11, 97, 210
+    M = rng.normal(scale=0.3, size=(o + v, o + v, P))
+    e = np.concatenate([np.sort(rng.uniform(-1, -0.2, o)), np.sort(rng.uniform(0, 1.5, v))])
+    rpa = px.RPA(dev_mmn(ctx, M))
+    rpa.configure(o - 1, 0, o + v - 1)
+    rpa.setRPAInputEnergies(e)
+    freqs = [0.0, 0.9]
+    monkeypatch.setenv("XTPGPU_BF16_KERNEL", kern)
+    g = rpa.calculate_epsilon(px.EPS_IMAG, freqs, "bf16x3")
+    for i, f in enumerate(freqs):
+        assert relerr(g[:, :, i], X.rpa_epsilon(M, e, o - 1, 0, o + v - 1, f, "imag")) < 2e-5
+
+
 @pytest.mark.parametrize("name,c", [("hqp", (1, 0, 0, 0)), ("hx", (0, 1, 0, 0)), ("hd", (0, 0, 1, 0)),
                                     ("hd2", (0, 0, 0, 1))])
 def test_bse_operator_fixtures_gpu(ctx, ints, name, c):  # test_bse_operator.cc:37-151
@@ -164,10 +182,11 @@ def test_bse_operator_random_mixed(ctx, c, prec, tol):
 
 @pytest.mark.parametrize("shape", [(11, 90, 130, 7), (40, 200, 300, 8)])
 @pytest.mark.parametrize("c", [(0, 0, 1, 0), (0, 0, 0, 1)])
-def test_bf16_pipelined_kernel_matches_staged(ctx, c, shape, monkeypatch):
-    """The direct-to-LDS split-BF16 GEMM (Hd / Hd2 stages) against the register-staged kernel and
-    FP64; the larger shape takes the 256x128 tile, the smaller the 128x128 one, both with ragged
-    edges (row and K extents that are not tile multiples)."""
+def test_bf16_kernels_agree(ctx, c, shape, monkeypatch):
+    """The three split-BF16 GEMM variants (register-staged with 32- and 64-wide K steps, and the
+    direct-to-LDS pipeline) on the Hd / Hd2 stages, against each other and FP64; the larger shape
+    takes the pipeline's 256x128 tile, the smaller its 128x128 one, both with ragged edges (row
+    and K extents that are not tile multiples)."""
     vt, ct, P, k = shape
     M, eps, Hqp, homo, rpamin, vmin, cmax = random_bse_case(14, vt=vt, ct=ct, P=P)
     t = dev_mmn(ctx, M)
@@ -175,13 +194,13 @@ def test_bf16_pipelined_kernel_matches_staged(ctx, c, shape, monkeypatch):
         px.BSEOperator_Options(homo, rpamin, 0, vmin, cmax))
     Xv = np.random.default_rng(15).normal(size=(op.size, k))
     ref = op.matmul(Xv, "fp64")
-    monkeypatch.setenv("XTPGPU_BF16_KERNEL", "pipe")
-    y_pipe = op.matmul(Xv, "bf16x3")
-    monkeypatch.setenv("XTPGPU_BF16_KERNEL", "staged")
-    y_staged = op.matmul(Xv, "bf16x3")
-    assert relerr(y_staged, ref) < 2e-5
-    assert relerr(y_pipe, ref) < 2e-5
-    assert relerr(y_pipe, y_staged) < 2e-5
+    ys = {}
+    for kern in ("staged32", "staged64", "pipe"):
+        monkeypatch.setenv("XTPGPU_BF16_KERNEL", kern)
+        ys[kern] = op.matmul(Xv, "bf16x3")
+        assert relerr(ys[kern], ref) < 2e-5, kern
+    assert relerr(ys["pipe"], ys["staged32"]) < 2e-5
+    assert relerr(ys["staged64"], ys["staged32"]) < 2e-5
 
 
 def test_davidson_fp64_matches_dense(ctx):

@@ -253,11 +253,15 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmDims d, Opnd<double> 
 // on load and re-split into hi/lo, so the weight costs no extra precision.
 // Block tile BM x BN x 32, 256 threads = 2x2 waves.
 // VEC: operands are K-contiguous with ld, base and K bounds multiples of 8 (16-byte loads).
-template <int BM, int BN, bool AK, bool BK_, bool SPLIT, bool HAS_S, bool VEC, class Epi>
+template <int BM, int BN, bool AK, bool BK_, bool SPLIT, bool HAS_S, bool VEC, class Epi, int BKT = 32>
 __device__ __forceinline__ void gemm_bf16_tile(const GemmDims& d, const Opnd<unsigned short>& A,
                                                const Opnd<unsigned short>& B, const float* __restrict__ s,
                                                long s_bs, const Epi& epi, int tm, int tn, int b, int ks) {
-  constexpr int BK = 32;
+  // BK = 64 (VEC only) gives every row a full 128-byte line per k-step: 8 lanes per row, 8 rows
+  // per load instruction; the padded row (72 elements = 144 B) keeps MFMA fragment reads
+  // conflict-free.
+  constexpr int BK = BKT;
+  static_assert(BK == 32 || (BK == 64 && VEC), "BK 64 needs vector loads");
   constexpr int PK = BK + 8;  // padded LDS row (bf16 elements), rows stay 16-byte aligned
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int TM = WM / 16, TN = WN / 16;
@@ -417,7 +421,9 @@ __device__ __forceinline__ void gemm_bf16_tile(const GemmDims& d, const Opnd<uns
       store();
       __syncthreads();
       if (k0 + BK < kend) load(k0 + BK);
-      const int kq = 8 * (lane >> 4);
+#pragma unroll
+      for (int kk = 0; kk < BK / 32; ++kk) {
+      const int kq = kk * 32 + 8 * (lane >> 4);
       bf8v ah[TM], bh[TN], al[TM], bl[TN];
 #pragma unroll
       for (int i = 0; i < TM; ++i) {
@@ -439,6 +445,7 @@ __device__ __forceinline__ void gemm_bf16_tile(const GemmDims& d, const Opnd<uns
             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
           }
         }
+      }
     }
   }
 #pragma unroll
@@ -454,12 +461,12 @@ __device__ __forceinline__ void gemm_bf16_tile(const GemmDims& d, const Opnd<uns
   __syncthreads();  // LDS is reused by the next tile of a persistent caller
 }
 
-template <int BM, int BN, bool AK, bool BK_, bool SPLIT, bool HAS_S, bool VEC, class Epi>
+template <int BM, int BN, bool AK, bool BK_, bool SPLIT, bool HAS_S, bool VEC, class Epi, int BKT = 32>
 __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmDims d, Opnd<unsigned short> A, Opnd<unsigned short> B,
                                                         const float* __restrict__ s, long s_bs, Epi epi) {
   int tm, tn;
   tile_of_block(d, tm, tn);
-  gemm_bf16_tile<BM, BN, AK, BK_, SPLIT, HAS_S, VEC, Epi>(d, A, B, s, s_bs, epi, tm, tn, blockIdx.z, blockIdx.y);
+  gemm_bf16_tile<BM, BN, AK, BK_, SPLIT, HAS_S, VEC, Epi, BKT>(d, A, B, s, s_bs, epi, tm, tn, blockIdx.z, blockIdx.y);
 }
 
 // ------------------------------------------------------------------ host launch helpers
@@ -512,12 +519,29 @@ inline hipError_t launch_gemm_f64(hipStream_t st, const GemmDims& d, Opnd<double
   return hipGetLastError();
 }
 
+// K-step of the vector-load path; d.kchunk must be a multiple of it (make_dims with this bk).
+inline bool bf16_vec(const Opnd<unsigned short>& A, const Opnd<unsigned short>& B, int K, long s_bs) {
+  return vec_ok(A, K, 8) && vec_ok(B, K, 8) && (s_bs % 8 == 0);
+}
+
 template <int BM, int BN, bool AK, bool BK_, class Epi>
 inline hipError_t launch_gemm_bf16(hipStream_t st, const GemmDims& d, Opnd<unsigned short> A,
                                    Opnd<unsigned short> B, const float* s, long s_bs, bool split,
-                                   Epi epi) {
+                                   Epi epi, bool bk64 = false) {
   dim3 grid(gemm_tiles(d), d.ksplit, d.batch);
-  const bool vec = AK && BK_ && vec_ok(A, d.K, 8) && vec_ok(B, d.K, 8) && (s_bs % 8 == 0);
+  const bool vec = AK && BK_ && bf16_vec(A, B, d.K, s_bs);
+  if constexpr (AK && BK_) {
+    if (vec && bk64) {
+      if (split) {
+        if (s) hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, AK, BK_, true, true, true, Epi, 64>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+        else hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, AK, BK_, true, false, true, Epi, 64>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+      } else {
+        if (s) hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, AK, BK_, false, true, true, Epi, 64>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+        else hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, AK, BK_, false, false, true, Epi, 64>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+      }
+      return hipGetLastError();
+    }
+  }
 #define XG_BF16_LAUNCH(SP, HS, VV) \
   hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, AK, BK_, SP, HS, VV, Epi>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi)
   if (vec) {

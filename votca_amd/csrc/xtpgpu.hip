@@ -134,19 +134,28 @@ void gemm64(hipStream_t st, int M, int N, int K, int batch, int ksplit, Opnd<dou
   XG_HIP((launch_gemm_f64<64, 64, AK, BK_>(st, d, A, B, s, s_bs, epi)));
 }
 
-// The direct-to-LDS pipelined kernel serves split, K-contiguous, unscaled products; the
-// register-staged kernel the rest. XTPGPU_BF16_KERNEL=staged forces the latter (A/B timing).
-inline bool pipe_enabled() {
+// BF16 kernel choice: 0 = register-staged BK 32, 1 = register-staged BK 64 (K-contiguous
+// operands), 2 = direct-to-LDS pipeline (split, K-contiguous, unscaled). XTPGPU_BF16_KERNEL
+// = staged32 | staged64 | pipe overrides the default for A/B timing.
+inline int bf16_kernel_choice() {
   const char* e = std::getenv("XTPGPU_BF16_KERNEL");
-  return !(e && std::strcmp(e, "staged") == 0);
+  if (e && std::strcmp(e, "staged32") == 0) return 0;
+  if (e && std::strcmp(e, "pipe") == 0) return 2;
+  return 1;
 }
 
 template <bool AK, bool BK_, class Epi>
 void gemm16(hipStream_t st, int M, int N, int K, int batch, int ksplit, Opnd<unsigned short> A,
             Opnd<unsigned short> B, const float* s, long s_bs, bool split, bool lower, Epi epi) {
   if (M <= 0 || N <= 0 || batch <= 0) return;
+  const int choice = bf16_kernel_choice();
   if constexpr (AK && BK_) {
-    if (split && !s && !lower && pipe_ok(A, B, K) && pipe_enabled()) {
+    if (choice == 1 && bf16_vec(A, B, K, s_bs)) {
+      GemmDims d = make_dims<128, 128>(M, N, std::max(K, 0), batch, ksplit, 64, lower);
+      XG_HIP((launch_gemm_bf16<128, 128, AK, BK_>(st, d, A, B, s, s_bs, split, epi, true)));
+      return;
+    }
+    if (choice == 2 && split && !s && !lower && pipe_ok(A, B, K)) {
       if ((long)M * N >= 256L * 128 * 512) {
         GemmDims d = make_dims<256, 128>(M, N, std::max(K, 0), batch, ksplit, 32, false);
         XG_HIP((launch_gemm_bf16x3_pipe<256, 128, 3>(st, d, A, B, epi)));
