@@ -217,14 +217,13 @@ def run_matvec(args):
             ", ".join(f"{nm} {t['ms']/args.steps:.2f} ms {t['flops']/max(t['ms']/1e3,1e-30)/1e12:.1f} TF"
                       for nm, t in tm.items()))
     if args.copy_calibration:
-        # a plain 1 GiB device-to-device copy kernel: the reference reading for the FETCH_SIZE /
-        # WRITE_SIZE counters (2^28 float32 read + 2^28 float32 written)
-        import torch
-        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        out["copy_calibration_bytes"] = {"read": 1 << 30, "written": 1 << 30}
+        # a plain 1 GiB device-to-device copy kernel (copy16_kernel): the known-size reading for
+        # the FETCH_SIZE / WRITE_SIZE counters (1 GiB read + 1 GiB written)
+        del X, Y
+        nb = 1 << 30
+        src, dst = px.DeviceBuffer(ctx, nb), px.DeviceBuffer(ctx, nb)
+        px._check(px.lib().xtpgpu_copy_device(ctx.h, dst.ptr, src.ptr, nb))
+        out["copy_calibration_bytes"] = {"read": nb, "written": nb}
     print(json.dumps(out))
 
 

@@ -66,6 +66,10 @@ int xtpgpu_malloc(xtpgpu_ctx ctx, int64_t bytes, void** dptr);
 int xtpgpu_free(xtpgpu_ctx ctx, void* dptr);
 int xtpgpu_memcpy_h2d(xtpgpu_ctx ctx, void* dst, const void* src, int64_t bytes);
 int xtpgpu_memcpy_d2h(xtpgpu_ctx ctx, void* dst, const void* src, int64_t bytes);
+/* Device-to-device copy by a plain streaming kernel (16-byte vectors; bytes and both pointers
+ * multiples of 16) on the context's stream. No counterpart in the reference: a utility for
+ * resident data, and the known-size reference kernel for HBM traffic counters. */
+int xtpgpu_copy_device(xtpgpu_ctx ctx, void* dst, const void* src, int64_t bytes);
 
 /* ---------------------------------------------------------------- TCMatrix_gwbse */
 /* TCMatrix_gwbse::Initialize (threecenter.cc:29-48): zero tensor, slices m in [mmin,mmax],

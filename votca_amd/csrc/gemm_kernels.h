@@ -41,6 +41,7 @@ struct GemmDims {
   int tiles_m, tiles_n;
   int lower;     // 1: only tiles with tile_n <= tile_m are computed (symmetric result)
   int nfast;     // 1: consecutive logical tiles walk along N (reuse of the A row block)
+  int group;     // > 0: grouped order, `group` row blocks x all column blocks at a time
 };
 
 // Block -> (tm, tn). Workgroups are dealt round-robin to the 8 XCDs (block b runs on
@@ -56,6 +57,14 @@ __device__ __forceinline__ void tile_of_block(const GemmDims& d, int& tm, int& t
     while ((tm + 1) * (tm + 2) / 2 <= tile) ++tm;
     while (tm * (tm + 1) / 2 > tile) --tm;
     tn = tile - tm * (tm + 1) / 2;
+  } else if (d.group > 0) {
+    // `group` row blocks sweep the column blocks together: the blocks resident on one XCD at a
+    // time share both their A row panels and their B column panels in that XCD's L2
+    const int span = d.group * d.tiles_n;
+    const int g = tile / span, w = tile - g * span;
+    const int gm = min(d.group, d.tiles_m - g * d.group);
+    tn = w / gm;
+    tm = g * d.group + (w - tn * gm);
   } else if (d.nfast) {
     tn = tile % d.tiles_n;
     tm = tile / d.tiles_n;
@@ -73,6 +82,55 @@ __device__ __forceinline__ unsigned short bf16_bits_rn(float f) {
 }
 __device__ __forceinline__ float bf16_bits_to_f(unsigned short h) {
   return __uint_as_float(((unsigned int)h) << 16);
+}
+
+// Epilogue protocol. Every epilogue is callable as epi(b, ks, m, n, v). A "separable" one
+// (kSeparable = true) also splits its output address into a row term and a column term:
+//   typename Epi::Row r = epi.row(b, ks, m);  long c = epi.col(b, n);  epi.put(r, c, v);
+// so a tile computes the (division-heavy) index arithmetic once per row and once per column
+// instead of once per element.
+template <class E, class = void>
+struct epi_separable { static constexpr bool value = false; };
+template <class E>
+struct epi_separable<E, decltype((void)E::kSeparable)> { static constexpr bool value = E::kSeparable; };
+
+// Store a (TM x TN) grid of MFMA result fragments through an epilogue. row_of(i, r) and
+// col_of(j) give the global row / column of fragment element (i, j, r).
+template <int TM, int TN, int NR, class Epi, class Acc, class RowOf, class ColOf>
+__device__ __forceinline__ void epilogue_store(const GemmDims& d, const Epi& epi, int b, int ks, const Acc& acc,
+                                               RowOf row_of, ColOf col_of) {
+  if constexpr (epi_separable<Epi>::value) {
+    long co[TN];
+    bool cv[TN];
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int gn = col_of(j);
+      cv[j] = gn < d.N;
+      co[j] = cv[j] ? epi.col(b, gn) : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const int gm = row_of(i, r);
+        if (gm < d.M) {
+          const typename Epi::Row ro = epi.row(b, ks, gm);
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            if (cv[j]) epi.put(ro, co[j], acc[i][j][r]);
+        }
+      }
+  } else {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          const int gm = row_of(i, r), gn = col_of(j);
+          if (gm < d.M && gn < d.N) epi(b, ks, gm, gn, acc[i][j][r]);
+        }
+  }
 }
 
 // ------------------------------------------------------------------ FP64 kernel
@@ -226,16 +284,9 @@ __device__ __forceinline__ void gemm_f64_tile(const GemmDims& d, const Opnd<doub
   }
   // D layout of 16x16x4 f64 (measured on gfx950): lane l, register r holds
   // D[(l/16) + 4r][l%16]; A[i][k] sits in lane i + 16k and B[k][j] in lane j + 16k.
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        int gm = m0 + wm0 + i * 16 + (lane >> 4) + 4 * r;
-        int gn = n0 + wn0 + j * 16 + (lane & 15);
-        if (gm < d.M && gn < d.N) epi(b, ks, gm, gn, acc[i][j][r]);
-      }
+  epilogue_store<TM, TN, 4>(
+      d, epi, b, ks, acc, [&](int i, int r) { return m0 + wm0 + i * 16 + (lane >> 4) + 4 * r; },
+      [&](int j) { return n0 + wn0 + j * 16 + (lane & 15); });
   __syncthreads();  // LDS is reused by the next tile of a persistent caller
 }
 
@@ -448,16 +499,9 @@ __device__ __forceinline__ void gemm_bf16_tile(const GemmDims& d, const Opnd<uns
       }
     }
   }
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        int gm = m0 + wm0 + i * 16 + 4 * (lane >> 4) + r;
-        int gn = n0 + wn0 + j * 16 + (lane & 15);
-        if (gm < d.M && gn < d.N) epi(b, ks, gm, gn, acc[i][j][r]);
-      }
+  epilogue_store<TM, TN, 4>(
+      d, epi, b, ks, acc, [&](int i, int r) { return m0 + wm0 + i * 16 + 4 * (lane >> 4) + r; },
+      [&](int j) { return n0 + wn0 + j * 16 + (lane & 15); });
   __syncthreads();  // LDS is reused by the next tile of a persistent caller
 }
 
@@ -485,6 +529,7 @@ inline GemmDims make_dims(int M, int N, int K, int batch, int ksplit, int bk, bo
   if (d.ksplit < 1) d.ksplit = 1;
   d.lower = lower ? 1 : 0;
   d.nfast = d.tiles_n <= d.tiles_m ? 1 : 0;
+  d.group = 0;
   return d;
 }
 
@@ -522,6 +567,23 @@ inline hipError_t launch_gemm_f64(hipStream_t st, const GemmDims& d, Opnd<double
 // K-step of the vector-load path; d.kchunk must be a multiple of it (make_dims with this bk).
 inline bool bf16_vec(const Opnd<unsigned short>& A, const Opnd<unsigned short>& B, int K, long s_bs) {
   return vec_ok(A, K, 8) && vec_ok(B, K, 8) && (s_bs % 8 == 0);
+}
+
+// 256 x 256 block tile (wave tile 128 x 128, one wave per SIMD), K step 32, vector loads:
+// half the operand bytes per MFMA of the 128 x 128 tile. The caller checks bf16_vec().
+template <bool AK, bool BK_, class Epi>
+inline hipError_t launch_gemm_bf16_big(hipStream_t st, const GemmDims& d, Opnd<unsigned short> A,
+                                       Opnd<unsigned short> B, const float* s, long s_bs, bool split, Epi epi) {
+  static_assert(AK && BK_, "K-contiguous operands only");
+  dim3 grid(gemm_tiles(d), d.ksplit, d.batch);
+  if (split) {
+    if (s) hipLaunchKernelGGL((gemm_bf16_kernel<256, 256, AK, BK_, true, true, true, Epi, 32>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+    else hipLaunchKernelGGL((gemm_bf16_kernel<256, 256, AK, BK_, true, false, true, Epi, 32>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+  } else {
+    if (s) hipLaunchKernelGGL((gemm_bf16_kernel<256, 256, AK, BK_, false, true, true, Epi, 32>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+    else hipLaunchKernelGGL((gemm_bf16_kernel<256, 256, AK, BK_, false, false, true, Epi, 32>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+  }
+  return hipGetLastError();
 }
 
 template <int BM, int BN, bool AK, bool BK_, class Epi>

@@ -157,16 +157,10 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_pipe_kernel(GemmDims d, Opnd<
   }
   // D layout of the 32x32 MFMA: lane l holds column l % 32, register r holds row
   // 8 (r / 4) + 4 (l / 32) + r % 4
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int gm = m0 + (wm * TM + i) * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        const int gn = n0 + (wn * TN + j) * 32 + (lane & 31);
-        if (gm < d.M && gn < d.N) epi(b, ks, gm, gn, acc[i][j][r]);
-      }
+  epilogue_store<TM, TN, 16>(
+      d, epi, b, ks, acc,
+      [&](int i, int r) { return m0 + (wm * TM + i) * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); },
+      [&](int j) { return n0 + (wn * TN + j) * 32 + (lane & 31); });
 }
 
 // Pipelined split-BF16 GEMM is usable when both operands are K-contiguous split planes with

@@ -72,6 +72,58 @@ __global__ __launch_bounds__(256) void colnorm_kernel(const double* __restrict__
   if (threadIdx.x == 0) out[blockIdx.x] = sqrt(red[0]);
 }
 
+// Deterministic column norms in two passes for tall columns: pass 1 gives partial sums of
+// squares over `nb` row chunks per column (grid nb x ncol), pass 2 reduces each column's nb
+// partials with one wave in a fixed butterfly order and takes the square root.
+__global__ __launch_bounds__(256) void colnorm_partial_kernel(const double* __restrict__ X, long ld, long n,
+                                                              double* __restrict__ part) {
+  __shared__ double red[256];
+  const int nb = gridDim.x;
+  const long chunk = (n + nb - 1) / nb;
+  const long lo = (long)blockIdx.x * chunk, hi = min(n, lo + chunk);
+  const double* x = X + (long)blockIdx.y * ld;
+  double s = 0.0;
+  for (long i = lo + threadIdx.x; i < hi; i += 256) s += x[i] * x[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[(long)blockIdx.y * nb + blockIdx.x] = red[0];
+}
+
+__device__ __forceinline__ double wave_sum(double s) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  return s;
+}
+
+// out[c] = sqrt(sum_b part[c*nb + b]); one wave per column, 4 columns per block.
+__global__ __launch_bounds__(256) void colnorm_finish_kernel(const double* __restrict__ part, int nb, long ncol,
+                                                             double* __restrict__ out) {
+  const long c = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (c >= ncol) return;
+  double s = 0.0;
+  for (int b = lane; b < nb; b += 64) s += part[c * nb + b];
+  s = wave_sum(s);
+  if (lane == 0) out[c] = sqrt(s);
+}
+
+// Sum of split-K partials, one wave per output (4 per block): lanes stride over the splits and
+// a fixed butterfly combines them, so the order is deterministic.
+__global__ __launch_bounds__(256) void reduce_partials_wave_kernel(const double* __restrict__ part, long stride,
+                                                                   int nsplit, long n, double* __restrict__ out) {
+  const long i = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int k = lane; k < nsplit; k += 64) s += part[(long)k * stride + i];
+  s = wave_sum(s);
+  if (lane == 0) out[i] = s;
+}
+
 // Sum of split-K partials: out[i] = sum_s part[s*stride + i] (fixed order).
 __global__ void reduce_partials_kernel(const double* __restrict__ part, long stride, int nsplit,
                                        long n, double* __restrict__ out) {

@@ -57,22 +57,32 @@ struct EpiStore {  // C[b*bs + m*sm + n*sn] = alpha*v + beta*C
   double* C;
   long sm, sn, bs;
   double alpha, beta;
+  static constexpr bool kSeparable = true;
+  typedef long Row;
+  __device__ long row(int b, int, int m) const { return (long)b * bs + (long)m * sm; }
+  __device__ long col(int, int n) const { return (long)n * sn; }
   template <typename V>
-  __device__ void operator()(int b, int, int m, int n, V v) const {
-    double* p = C + (long)b * bs + (long)m * sm + (long)n * sn;
-    double r = alpha * (double)v;
-    if (beta != 0.0) r += beta * (*p);
-    *p = r;
+  __device__ void put(long r, long c, V v) const {
+    double* p = C + r + c;
+    double x = alpha * (double)v;
+    if (beta != 0.0) x += beta * (*p);
+    *p = x;
   }
+  template <typename V>
+  __device__ void operator()(int b, int ks, int m, int n, V v) const { put(row(b, ks, m), col(b, n), v); }
 };
 
 struct EpiPartial {  // split-K partials: part[(ks*batch + b)*M*N + n*M + m]
   double* part;
   int M, N, batch;
+  static constexpr bool kSeparable = true;
+  typedef long Row;
+  __device__ long row(int b, int ks, int m) const { return ((long)ks * batch + b) * (long)M * N + m; }
+  __device__ long col(int, int n) const { return (long)n * M; }
   template <typename V>
-  __device__ void operator()(int b, int ks, int m, int n, V v) const {
-    part[((long)ks * batch + b) * (long)M * N + (long)n * M + m] = (double)v;
-  }
+  __device__ void put(long r, long c, V v) const { part[r + c] = (double)v; }
+  template <typename V>
+  __device__ void operator()(int b, int ks, int m, int n, V v) const { put(row(b, ks, m), col(b, n), v); }
 };
 
 struct EpiEpsSym {  // eps[b](m,n) = eps[b](n,m) = v + delta_mn, written once from m >= n
@@ -99,14 +109,30 @@ struct EpiStage1 {
   long p0;
   int rows, cols, colsp, kvec, pc;  // colsp: padded row length of the K-contiguous output
   bool by_row;
+  // idx = outer * pc * colsp + p * colsp + j with outer = kv * rows + r (or r * kvec + kv):
+  // the row (p, r) and column (kv, j) terms separate.
+  static constexpr bool kSeparable = true;
+  struct Row {
+    long off;
+    double f;
+  };
+  __device__ Row row(int, int, int m) const {
+    const int p = m / rows, r = m - p * rows;
+    const long slab = (long)pc * colsp;
+    Row o;
+    o.off = (by_row ? (long)r * kvec * slab : (long)r * slab) + (long)p * colsp;
+    o.f = scale ? scale[p0 + p] : 1.0;
+    return o;
+  }
+  __device__ long col(int, int n) const {
+    const int kv = n / cols, j = n - kv * cols;
+    const long slab = (long)pc * colsp;
+    return (by_row ? (long)kv * slab : (long)kv * rows * slab) + j;
+  }
   template <typename V>
-  __device__ void operator()(int, int, int m, int n, V v) const {
-    int p = m / rows, r = m - p * rows;
-    int kv = n / cols, j = n - kv * cols;
-    long outer = by_row ? ((long)r * kvec + kv) : ((long)kv * rows + r);
-    long idx = outer * ((long)pc * colsp) + (long)p * colsp + j;
-    double x = (double)v;
-    if (scale) x *= scale[p0 + p];
+  __device__ void put(const Row& ro, long c, V v) const {
+    const long idx = ro.off + c;
+    const double x = (double)v * ro.f;
     if (S64) {
       S64[idx] = x;
     } else {
@@ -116,9 +142,19 @@ struct EpiStage1 {
       if (Sl) Sl[idx] = l;
     }
   }
+  template <typename V>
+  __device__ void operator()(int b, int ks, int m, int n, V v) const { put(row(b, ks, m), col(b, n), v); }
 };
 
 // ------------------------------------------------------------------ GEMM dispatch
+// Row-block group for the grouped tile order (0 = plain order): used when both tile counts
+// are large enough for panels to be evicted between reuses. XTPGPU_TILE_GROUP overrides.
+inline int tile_group(const GemmDims& d) {
+  const char* e = std::getenv("XTPGPU_TILE_GROUP");
+  if (e) return std::max(0, std::atoi(e));
+  return (d.tiles_m >= 64 && d.tiles_n >= 8) ? 8 : 0;
+}
+
 template <bool AK, bool BK_, class Epi>
 void gemm64(hipStream_t st, int M, int N, int K, int batch, int ksplit, Opnd<double> A, Opnd<double> B,
             const double* s, long s_bs, bool lower, Epi epi) {
@@ -127,6 +163,7 @@ void gemm64(hipStream_t st, int M, int N, int K, int batch, int ksplit, Opnd<dou
   const long big_tiles = (long)((M + 127) / 128) * ((N + 127) / 128) * batch * std::max(ksplit, 1);
   if (ksplit <= 1 && M >= 256 && N >= 256 && big_tiles >= 512) {
     GemmDims d = make_dims<128, 128>(M, N, std::max(K, 0), batch, ksplit, 16, lower);
+    if (!lower) d.group = tile_group(d);
     XG_HIP((launch_gemm_f64<128, 128, AK, BK_>(st, d, A, B, s, s_bs, epi)));
     return;
   }
@@ -141,6 +178,7 @@ inline int bf16_kernel_choice() {
   const char* e = std::getenv("XTPGPU_BF16_KERNEL");
   if (e && std::strcmp(e, "staged32") == 0) return 0;
   if (e && std::strcmp(e, "pipe") == 0) return 2;
+  if (e && std::strcmp(e, "big") == 0) return 3;
   return 1;
 }
 
@@ -150,8 +188,14 @@ void gemm16(hipStream_t st, int M, int N, int K, int batch, int ksplit, Opnd<uns
   if (M <= 0 || N <= 0 || batch <= 0) return;
   const int choice = bf16_kernel_choice();
   if constexpr (AK && BK_) {
-    if (choice == 1 && bf16_vec(A, B, K, s_bs)) {
+    if (choice == 3 && bf16_vec(A, B, K, s_bs) && (long)M * N >= 256L * 256 * 512) {
+      GemmDims d = make_dims<256, 256>(M, N, std::max(K, 0), batch, ksplit, 32, lower);
+      XG_HIP((launch_gemm_bf16_big<AK, BK_>(st, d, A, B, s, s_bs, split, epi)));
+      return;
+    }
+    if ((choice == 1 || choice == 3) && bf16_vec(A, B, K, s_bs)) {
       GemmDims d = make_dims<128, 128>(M, N, std::max(K, 0), batch, ksplit, 64, lower);
+      if (!lower) d.group = tile_group(d);
       XG_HIP((launch_gemm_bf16<128, 128, AK, BK_>(st, d, A, B, s, s_bs, split, epi, true)));
       return;
     }
@@ -170,10 +214,11 @@ void gemm16(hipStream_t st, int M, int N, int K, int batch, int ksplit, Opnd<uns
   XG_HIP((launch_gemm_bf16<128, 128, AK, BK_>(st, d, A, B, s, s_bs, split, epi)));
 }
 
-// Number of K splits so that tiles*splits fills the device several times over.
+// Number of K splits so that tiles*splits gives about four workgroups per CU, with K chunks
+// of at least 512 so each workgroup amortises its launch and epilogue.
 int pick_ksplit(long tiles, long K, int bk) {
-  long want = std::max<long>(1, 2048 / std::max<long>(1, tiles));
-  long maxs = std::max<long>(1, K / (4 * bk));
+  long want = std::max<long>(1, 1024 / std::max<long>(1, tiles));
+  long maxs = std::max<long>(1, K / std::max(512, 4 * bk));
   return (int)std::max<long>(1, std::min(want, maxs));
 }
 
@@ -298,6 +343,28 @@ extern "C" int xtpgpu_memcpy_d2h(xtpgpu_ctx ctx, void* dst, const void* src, int
   return guarded([&] {
     XG_REQUIRE(ctx && bytes >= 0, XTPGPU_ERR_ARG, "bad arguments");
     XG_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, ctx->stream));
+    XG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+namespace {
+__global__ __launch_bounds__(256) void copy16_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src,
+                                                     size_t n) {
+  for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" int xtpgpu_copy_device(xtpgpu_ctx ctx, void* dst, const void* src, int64_t bytes) {
+  return guarded([&] {
+    XG_REQUIRE(ctx && dst && src && bytes >= 0 && bytes % 16 == 0 && (uintptr_t)dst % 16 == 0 &&
+                   (uintptr_t)src % 16 == 0,
+               XTPGPU_ERR_ARG, "xtpgpu_copy_device: pointers and size must be 16-byte multiples");
+    XG_HIP(hipSetDevice(ctx->device));
+    const size_t n = (size_t)bytes / 16;
+    if (n == 0) return;
+    const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 16);
+    hipLaunchKernelGGL(copy16_kernel, dim3(grid), dim3(256), 0, ctx->stream, (uint4*)dst, (const uint4*)src, n);
+    XG_HIP(hipGetLastError());
     XG_HIP(hipStreamSynchronize(ctx->stream));
   });
 }

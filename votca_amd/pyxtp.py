@@ -69,6 +69,7 @@ SIGNATURES = {
     "xtpgpu_free": (C.c_int, [_vp, _vp]),
     "xtpgpu_memcpy_h2d": (C.c_int, [_vp, _vp, _vp, _i64]),
     "xtpgpu_memcpy_d2h": (C.c_int, [_vp, _vp, _vp, _i64]),
+    "xtpgpu_copy_device": (C.c_int, [_vp, _vp, _vp, _i64]),
     "xtpgpu_mmn_create": (C.c_int, [_vp, _i64, _i64, _i64, _i64, _i64, C.POINTER(_vp)]),
     "xtpgpu_mmn_destroy": (C.c_int, [_vp]),
     "xtpgpu_mmn_set_slice": (C.c_int, [_vp, _i64, _dp, _i64]),
