@@ -227,6 +227,106 @@ def run_matvec(args):
     print(json.dumps(out))
 
 
+def run_adjoint(args):
+    """Full (non-TDA) singlet BSE for `--roots` roots, then the Hellmann-Feynman derivatives of the
+    lowest `--adj-roots` with respect to the QP energies (all levels) and the transition block
+    M^P_{vc} (all P, v, c) at fixed screening, checked against central finite differences
+    (h = `--fd-h` Ha) on `--fd-levels` seeded QP levels and `--fd-mmn` seeded Mmn entries; each
+    finite difference re-solves the perturbed problem twice in FP64."""
+    from votca_amd import pyxtp as px
+    o, v, P = args.nocc, args.nvirt, args.naux
+    n = o + v
+    ctx = px.Context(0)
+    log(f"[adjoint] o={o} v={v} P={P}, {args.roots} roots (full BSE), derivatives of {args.adj_roots}")
+    Mmn = px.TCMatrix_gwbse(ctx).Initialize(P, 0, n - 1, 0, n - 1)
+    Mmn.fill_synthetic(args.seed, args.sigma0, 3.0)
+    e = synthetic_energies(args.seed, o, v)
+    rpa = px.RPA(Mmn)
+    rpa.configure(o - 1, 0, n - 1)
+    rpa.setRPAInputEnergies(e)
+    lam, U = np.linalg.eigh(rpa.calculate_epsilon_r(0.0))
+    Mmn.MultiplyRightWithAuxMatrix(U)
+    epsinv = np.where(lam > 1e-8, 1.0 / lam, 0.0)
+    opt = px.BSEOperator_Options(o - 1, 0, 0, 0, n - 1)
+
+    def solve(Hqp):
+        A = px.SingletOperator_TDA(epsinv, Mmn, Hqp).configure(opt)
+        B = px.SingletOperator_BTDA_B(epsinv, Mmn, Hqp).configure(opt)
+        ds = px.DavidsonSolver()
+        ds.set_tolerance(args.tol)
+        ds.set_correction("DPR")
+        ds.set_size_update("safe")
+        ds.set_max_search_space(10 * args.roots)
+        ds.set_iter_max(args.iter_max)
+        t0 = time.perf_counter()
+        ev, Xv, Yv = ds.solve_full(A, B, args.roots)
+        wall = time.perf_counter() - t0
+        if not ds.converged():
+            raise RuntimeError(f"full BSE did not converge (residual {ds.info.max_residual:.2e})")
+        return A, B, ev, Xv, Yv, wall, ds.info
+
+    H0 = np.diag(e)
+    A, B, ev, Xv, Yv, wall, info = solve(H0)
+    log(f"[adjoint] full BSE: {wall:.2f} s, {info.iterations} it, residual {info.max_residual:.1e}")
+    na = args.adj_roots
+    N = A.size
+    vt, ct = A.vt, A.ct
+    d_eps = np.zeros((vt + ct, na), order="F")
+    dm = px.DeviceBuffer(ctx, na * P * N * 8)
+    Xa, Ya = np.asfortranarray(Xv[:, :na]), np.asfortranarray(Yv[:, :na])
+    px._check(px.lib().xtpgpu_bse_adjoint_device(A.h, B.h, na, px._dptr(Xa), px._dptr(Ya), N, px._dptr(d_eps),
+                                                 dm.ptr))
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    px._check(px.lib().xtpgpu_bse_adjoint_device(A.h, B.h, na, px._dptr(Xa), px._dptr(Ya), N, px._dptr(d_eps),
+                                                 dm.ptr))
+    ctx.synchronize()
+    t_adj = time.perf_counter() - t0
+    log(f"[adjoint] d/d(e_QP) and d/dM for {na} roots: {t_adj*1e3:.1f} ms ({na*P*N*8/1e9:.2f} GB of dOmega/dM)")
+    A.close(), B.close()
+    rng = np.random.default_rng(args.seed + 1)
+    h = args.fd_h
+    checks = []
+    for lvl in rng.choice(vt + ct, args.fd_levels, replace=False):
+        w = []
+        for sgn in (1.0, -1.0):
+            Hp = H0.copy()
+            Hp[lvl, lvl] += sgn * h
+            a_, b_, evp, *_ = solve(Hp)
+            a_.close(), b_.close()
+            w.append(evp[:na])
+        fd = (w[0] - w[1]) / (2 * h)
+        checks.append(("e_QP", int(lvl), fd, d_eps[lvl, :na].copy()))
+    # d/dM^P_{vc}: the adjoint's layout is [root][P][v][c]
+    for _ in range(args.fd_mmn):
+        p_, v_, c_ = int(rng.integers(P)), int(rng.integers(vt)), int(rng.integers(ct))
+        col = Mmn[v_][o + c_, :].copy()          # M^P_{v, c} for all P (slice v, row c)
+        w = []
+        for sgn in (1.0, -1.0):
+            m = col.copy()
+            m[p_] += sgn * h
+            Mmn.set_block(v_, o + c_, m.reshape(P, 1, 1), symmetric=True)
+            a_, b_, evp, *_ = solve(H0)
+            a_.close(), b_.close()
+            w.append(evp[:na])
+        Mmn.set_block(v_, o + c_, col.reshape(P, 1, 1), symmetric=True)
+        fd = (w[0] - w[1]) / (2 * h)
+        adj = np.array([dm.download((1,), offset=((r * P + p_) * N + v_ * ct + c_) * 8)[0] for r in range(na)])
+        checks.append(("M", (p_, v_, c_), fd, adj))
+    worst = 0.0
+    rows = []
+    for kind, where, fd, adj in checks:
+        scale = max(np.abs(fd).max(), 1e-12)
+        rel = float(np.abs(fd - adj).max() / scale)
+        worst = max(worst, rel)
+        rows.append({"param": kind, "index": where, "fd": [float(x) for x in fd], "adjoint": [float(x) for x in adj],
+                     "max_rel_err": rel})
+        log(f"[adjoint] {kind} {where}: max rel err {rel:.2e}")
+    print(json.dumps({"workload": f"full BSE + adjoint, n_occ={o} n_virt={v} n_aux={P}, {args.roots} roots",
+                      "solve_s": wall, "iterations": info.iterations, "adjoint_s": t_adj, "adjoint_roots": na,
+                      "dOmega_dM_bytes": na * P * N * 8, "fd_h": h, "max_rel_err": worst, "checks": rows}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -246,7 +346,11 @@ def main():
                          "on which the reference Davidson does not converge (exchange ~100x physical)")
     ap.add_argument("--iter-max", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="single", choices=["single", "segments", "matvec"])
+    ap.add_argument("--workload", default="single", choices=["single", "segments", "matvec", "adjoint"])
+    ap.add_argument("--adj-roots", type=int, default=5)
+    ap.add_argument("--fd-levels", type=int, default=3)
+    ap.add_argument("--fd-mmn", type=int, default=3)
+    ap.add_argument("--fd-h", type=float, default=1e-5)
     ap.add_argument("--cols", type=int, default=30)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="matvec workload: also run a 1 GiB copy kernel (counter calibration)")
@@ -264,6 +368,9 @@ def main():
         return
     if args.workload == "matvec":
         run_matvec(args)
+        return
+    if args.workload == "adjoint":
+        run_adjoint(args)
         return
     from votca_amd import pyxtp as px
     from votca_amd import quadrature
@@ -373,7 +480,10 @@ def main():
     tm = res["mixed"]["hd_stage1"]
     kt = tm["ms"] / 1e3 / max(1, tm["launches"])
     ach = tm["flops"] / max(tm["ms"] / 1e3, 1e-30)
-    roofline = {"bound": "mfma", "kernel": "hd_stage1 (gemm_bf16x3_pipe_kernel<256,128,3,EpiStage1>)",
+    kname = {"staged32": "gemm_bf16_kernel<128,128,..,EpiStage1,32>", "pipe": "gemm_bf16x3_pipe_kernel<..,EpiStage1>",
+             "big": "gemm_bf16_kernel<256,256,..,EpiStage1,32>"}.get(os.environ.get("XTPGPU_BF16_KERNEL", ""),
+                                                                    "gemm_bf16_kernel<128,128,..,EpiStage1,64>")
+    roofline = {"bound": "mfma", "kernel": f"hd_stage1 ({kname})",
                 "achieved": ach / 1e12,
                 "peak": PEAK["bf16"] / 1e12, "unit": "TFLOP/s", "frac": ach / PEAK["bf16"], "traffic": None,
                 "avg_launch_ms": kt * 1e3, "algorithmic_flops_per_launch": tm["flops"] / max(1, tm["launches"])}

@@ -216,9 +216,12 @@ class DeviceBuffer:
         assert a.nbytes <= self.nbytes
         _check(lib().xtpgpu_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data_as(_vp), a.nbytes))
 
-    def download(self, shape, dtype=np.float64, order="F") -> np.ndarray:
+    def download(self, shape, dtype=np.float64, order="F", offset=0) -> np.ndarray:
+        """Copy `shape` elements starting `offset` bytes into the buffer back to the host."""
         out = np.empty(shape, dtype=dtype, order=order)
-        _check(lib().xtpgpu_memcpy_d2h(self.ctx.h, out.ctypes.data_as(_vp), self.ptr, out.nbytes))
+        assert offset >= 0 and offset + out.nbytes <= self.nbytes
+        src = C.c_void_p(self.ptr.value + offset)
+        _check(lib().xtpgpu_memcpy_d2h(self.ctx.h, out.ctypes.data_as(_vp), src, out.nbytes))
         return out
 
     def free(self):
