@@ -32,6 +32,12 @@ sys.path.insert(0, ROOT)
 
 HRT2EV = 27.21138602
 PEAK = {"fp64": 78.6e12, "bf16": 2.5e15, "hbm": 8.0e12}
+# HBM bytes of one hd_stage1 launch of the split-BF16 kernel (30-column H X, P-chunk of 263 levels,
+# 12 355 584 work-items), from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of
+# `bench.py --workload matvec --steps 1 --copy-calibration`, each scaled by what the same counter
+# reads for the 1 GiB copy16_kernel in that run (FETCH_SIZE 0.5 GiB -> x2.0, WRITE_SIZE 1.0 GiB -> x1.0).
+HD_STAGE1_TRAFFIC = {"read": 4.078 * 2.0 * 2**30, "write": 3.311 * 1.0 * 2**30,
+                     "launch": "30-column H X stage-1 launch (P-chunk 263)"}
 
 
 def log(*a):
@@ -209,7 +215,8 @@ def run_matvec(args):
             op.matmul_device(X, k, Y, prec)
         ctx.synchronize()
         wall = (time.perf_counter() - t0) / args.steps
-        tm = {name: ctx.timer(name) for name in ("hd_stage1", "hd_stage2", "hx")}
+        sfx = "_fp64" if prec == "fp64" else ""
+        tm = {name: ctx.timer(name + (sfx if name != "hx" else "")) for name in ("hd_stage1", "hd_stage2", "hx")}
         out[prec] = {"wall_ms": wall * 1e3,
                      **{f"{name}_tflops": t["flops"] / max(t["ms"] / 1e3, 1e-30) / 1e12 for name, t in tm.items()},
                      **{f"{name}_ms": t["ms"] / max(1, args.steps) for name, t in tm.items()}}
@@ -445,7 +452,7 @@ def main():
         ev, _ = ds.solve(op, args.roots, want_vectors=False)
         wall = time.perf_counter() - t0
         info = ds.info
-        tm = ctx.timer("hd_stage1")
+        tm = ctx.timer("hd_stage1_fp64" if prec == "fp64" else "hd_stage1")
         return ev, wall, info, tm
 
     res = {}
@@ -483,9 +490,11 @@ def main():
     kname = {"staged32": "gemm_bf16_kernel<128,128,..,EpiStage1,32>", "pipe": "gemm_bf16x3_pipe_kernel<..,EpiStage1>",
              "big": "gemm_bf16_kernel<256,256,..,EpiStage1,32>"}.get(os.environ.get("XTPGPU_BF16_KERNEL", ""),
                                                                     "gemm_bf16_kernel<128,128,..,EpiStage1,64>")
+    traffic = HD_STAGE1_TRAFFIC["read"] + HD_STAGE1_TRAFFIC["write"] if kname.endswith(",64>") else None
     roofline = {"bound": "mfma", "kernel": f"hd_stage1 ({kname})",
                 "achieved": ach / 1e12,
-                "peak": PEAK["bf16"] / 1e12, "unit": "TFLOP/s", "frac": ach / PEAK["bf16"], "traffic": None,
+                "peak": PEAK["bf16"] / 1e12, "unit": "TFLOP/s", "frac": ach / PEAK["bf16"], "traffic": traffic,
+                "traffic_launch": HD_STAGE1_TRAFFIC["launch"] if traffic else None,
                 "avg_launch_ms": kt * 1e3, "algorithmic_flops_per_launch": tm["flops"] / max(1, tm["launches"])}
 
     cpu = None

@@ -54,9 +54,10 @@ int xtpgpu_ctx_create(int device, unsigned flags, xtpgpu_ctx* out);
 int xtpgpu_ctx_destroy(xtpgpu_ctx ctx);
 int xtpgpu_ctx_synchronize(xtpgpu_ctx ctx);
 /* Live kernel timing: when enabled, HIP events are recorded on the context's stream around
- * the dominant launches (names: "rpa_eps_gemm", "hd_stage1", "hd_stage2", "hd2_stage1",
- * "hd2_stage2", "hx", "hqp", "davidson_ortho"), together with each launch's algorithmic
- * FLOPs and bytes. xtpgpu_ctx_timer synchronizes the stream and returns the totals. */
+ * the dominant launches (names: "rpa_eps_gemm", "hd_stage1" / "hd_stage2" for the BF16 forms
+ * and "hd_stage1_fp64" / "hd_stage2_fp64" for FP64, "hd2", "hx", "hqp"), together with each
+ * launch's algorithmic FLOPs and bytes. xtpgpu_ctx_timer synchronizes the stream and returns
+ * the totals. */
 int xtpgpu_ctx_set_profiling(xtpgpu_ctx ctx, int enable);
 int xtpgpu_ctx_timer(xtpgpu_ctx ctx, const char* name, double* total_ms, int64_t* launches,
                      double* flops, double* bytes);

@@ -826,12 +826,12 @@ void bse_matmul_impl(xtpgpu_bseop op, int precision, const double* X, long ldx, 
         double* S = op->s1.as<double>();
         e1.S64 = S;
         {
-          ProfScope prof(ctx, "hd_stage1", f1, b1);
+          ProfScope prof(ctx, "hd_stage1_fp64", f1, b1);
           gemm64<true, true>(st, (int)M1, (int)N1, (int)ctp, 1, 1,
                              Opnd<double>{op->Mcc.as<double>() + p0 * ct * ctp, nullptr, ctp, 0},
                              Opnd<double>{Xp, nullptr, ctp, 0}, nullptr, 0, false, e1);
         }
-        ProfScope prof(ctx, "hd_stage2", f2, b2);
+        ProfScope prof(ctx, "hd_stage2_fp64", f2, b2);
         gemm64<true, true>(st, (int)ct, (int)vt, (int)(np * vtp), (int)k, 1,
                            Opnd<double>{S, nullptr, k * np * vtp, np * vtp},
                            Opnd<double>{op->Bvv.as<double>() + p0 * vtp, nullptr, P * vtp, 0}, nullptr, 0, false,
