@@ -322,6 +322,13 @@ def run_adjoint(args):
         checks.append(("M", (p_, v_, c_), fd, adj))
     worst = 0.0
     rows = []
+    fd_all = np.concatenate([c[2] for c in checks]) if checks else np.zeros(1)
+    adj_all = np.concatenate([c[3] for c in checks]) if checks else np.zeros(1)
+    normwise = float(np.linalg.norm(adj_all - fd_all) / max(np.linalg.norm(fd_all), 1e-300))
+    # rounding floor of a central difference: a few ulps of Omega over 2h
+    noise = float(4 * np.finfo(float).eps * np.abs(ev[:na]).max() / (2 * h))
+    log(f"[adjoint] norm-wise relative error over all checks {normwise:.2e}; "
+        f"finite-difference rounding floor ~{noise:.1e} (absolute)")
     for kind, where, fd, adj in checks:
         scale = max(np.abs(fd).max(), 1e-12)
         rel = float(np.abs(fd - adj).max() / scale)
@@ -331,7 +338,8 @@ def run_adjoint(args):
         log(f"[adjoint] {kind} {where}: max rel err {rel:.2e}")
     print(json.dumps({"workload": f"full BSE + adjoint, n_occ={o} n_virt={v} n_aux={P}, {args.roots} roots",
                       "solve_s": wall, "iterations": info.iterations, "adjoint_s": t_adj, "adjoint_roots": na,
-                      "dOmega_dM_bytes": na * P * N * 8, "fd_h": h, "max_rel_err": worst, "checks": rows}))
+                      "dOmega_dM_bytes": na * P * N * 8, "fd_h": h, "max_rel_err": worst,
+                      "normwise_rel_err": normwise, "fd_rounding_floor": noise, "checks": rows}))
 
 
 def main():

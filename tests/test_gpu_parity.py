@@ -195,7 +195,7 @@ def test_bf16_kernels_agree(ctx, c, shape, monkeypatch):
     Xv = np.random.default_rng(15).normal(size=(op.size, k))
     ref = op.matmul(Xv, "fp64")
     ys = {}
-    for kern in ("staged32", "staged64", "pipe"):
+    for kern in ("staged32", "staged64", "pipe", "big", "tall"):
         monkeypatch.setenv("XTPGPU_BF16_KERNEL", kern)
         ys[kern] = op.matmul(Xv, "bf16x3")
         assert relerr(ys[kern], ref) < 2e-5, kern

@@ -302,9 +302,9 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmDims d, Opnd<double> 
 // Operands are bf16 bit patterns (unsigned short). SPLIT: use hi/lo parts of both
 // operands (3 MFMAs per fragment pair). HAS_S: B(k,n) is rescaled by s[k] (float)
 // on load and re-split into hi/lo, so the weight costs no extra precision.
-// Block tile BM x BN x 32, 256 threads = 2x2 waves.
+// Block tile BM x BN x BKT, NT threads = (NT/128) x 2 waves.
 // VEC: operands are K-contiguous with ld, base and K bounds multiples of 8 (16-byte loads).
-template <int BM, int BN, bool AK, bool BK_, bool SPLIT, bool HAS_S, bool VEC, class Epi, int BKT = 32>
+template <int BM, int BN, bool AK, bool BK_, bool SPLIT, bool HAS_S, bool VEC, class Epi, int BKT = 32, int NT = 256>
 __device__ __forceinline__ void gemm_bf16_tile(const GemmDims& d, const Opnd<unsigned short>& A,
                                                const Opnd<unsigned short>& B, const float* __restrict__ s,
                                                long s_bs, const Epi& epi, int tm, int tn, int b, int ks) {
@@ -314,11 +314,12 @@ __device__ __forceinline__ void gemm_bf16_tile(const GemmDims& d, const Opnd<uns
   constexpr int BK = BKT;
   static_assert(BK == 32 || (BK == 64 && VEC), "BK 64 needs vector loads");
   constexpr int PK = BK + 8;  // padded LDS row (bf16 elements), rows stay 16-byte aligned
-  constexpr int WM = BM / 2, WN = BN / 2;
+  static_assert(NT == 256 || NT == 512, "256 or 512 threads");
+  constexpr int WM = BM / (NT / 128), WN = BN / 2;
   constexpr int TM = WM / 16, TN = WN / 16;
   constexpr int NS = SPLIT ? 2 : 1;
-  constexpr int LA = BM * BK / 256;
-  constexpr int LB = BN * BK / 256;
+  constexpr int LA = BM * BK / NT;
+  constexpr int LB = BN * BK / NT;
   __shared__ __attribute__((aligned(16))) unsigned short As[NS][BM * PK];
   __shared__ __attribute__((aligned(16))) unsigned short Bs[NS][BN * PK];
 
@@ -345,14 +346,14 @@ __device__ __forceinline__ void gemm_bf16_tile(const GemmDims& d, const Opnd<uns
     for (int j = 0; j < TN; ++j) acc[i][j] = (f4v){0.f, 0.f, 0.f, 0.f};
 
   unsigned short ra[NS][LA], rb[NS][LB];
-  constexpr int VA = BM * BK / 8 / 256;  // 16-byte vectors per thread (VEC path)
-  constexpr int VB = BN * BK / 8 / 256;
+  constexpr int VA = BM * BK / 8 / NT;  // 16-byte vectors per thread (VEC path)
+  constexpr int VB = BN * BK / 8 / NT;
   us8v va[NS][VA > 0 ? VA : 1], vb[NS][VB > 0 ? VB : 1];
 
   auto load_vec = [&](int k0) {
 #pragma unroll
     for (int i = 0; i < VA; ++i) {
-      int vid = tid + i * 256;
+      int vid = tid + i * NT;
       int mm = vid / (BK / 8), kq = 8 * (vid % (BK / 8));
       int gm = m0 + mm, gk = k0 + kq;
       us8v h = {0, 0, 0, 0, 0, 0, 0, 0}, l = h;
@@ -366,7 +367,7 @@ __device__ __forceinline__ void gemm_bf16_tile(const GemmDims& d, const Opnd<uns
     }
 #pragma unroll
     for (int i = 0; i < VB; ++i) {
-      int vid = tid + i * 256;
+      int vid = tid + i * NT;
       int nn = vid / (BK / 8), kq = 8 * (vid % (BK / 8));
       int gn = n0 + nn, gk = k0 + kq;
       us8v h = {0, 0, 0, 0, 0, 0, 0, 0}, l = h;
@@ -393,13 +394,13 @@ __device__ __forceinline__ void gemm_bf16_tile(const GemmDims& d, const Opnd<uns
     for (int q = 0; q < NS; ++q) {
 #pragma unroll
       for (int i = 0; i < VA; ++i) {
-        int vid = tid + i * 256;
+        int vid = tid + i * NT;
         int mm = vid / (BK / 8), kq = 8 * (vid % (BK / 8));
         *reinterpret_cast<us8v*>(&As[q][mm * PK + kq]) = va[q][i];
       }
 #pragma unroll
       for (int i = 0; i < VB; ++i) {
-        int vid = tid + i * 256;
+        int vid = tid + i * NT;
         int nn = vid / (BK / 8), kq = 8 * (vid % (BK / 8));
         *reinterpret_cast<us8v*>(&Bs[q][nn * PK + kq]) = vb[q][i];
       }
@@ -410,7 +411,7 @@ __device__ __forceinline__ void gemm_bf16_tile(const GemmDims& d, const Opnd<uns
     if (VEC) { load_vec(k0); return; }
 #pragma unroll
     for (int i = 0; i < LA; ++i) {
-      int idx = tid + i * 256;
+      int idx = tid + i * NT;
       int mm, kk;
       if (AK) { mm = idx / BK; kk = idx % BK; } else { mm = idx % BM; kk = idx / BM; }
       int gm = m0 + mm, gk = k0 + kk;
@@ -425,7 +426,7 @@ __device__ __forceinline__ void gemm_bf16_tile(const GemmDims& d, const Opnd<uns
     }
 #pragma unroll
     for (int i = 0; i < LB; ++i) {
-      int idx = tid + i * 256;
+      int idx = tid + i * NT;
       int nn, kk;
       if (BK_) { nn = idx / BK; kk = idx % BK; } else { nn = idx % BN; kk = idx / BN; }
       int gn = n0 + nn, gk = k0 + kk;
@@ -450,14 +451,14 @@ __device__ __forceinline__ void gemm_bf16_tile(const GemmDims& d, const Opnd<uns
     for (int q = 0; q < NS; ++q) {
 #pragma unroll
       for (int i = 0; i < LA; ++i) {
-        int idx = tid + i * 256;
+        int idx = tid + i * NT;
         int mm, kk;
         if (AK) { mm = idx / BK; kk = idx % BK; } else { mm = idx % BM; kk = idx / BM; }
         As[q][mm * PK + kk] = ra[q][i];
       }
 #pragma unroll
       for (int i = 0; i < LB; ++i) {
-        int idx = tid + i * 256;
+        int idx = tid + i * NT;
         int nn, kk;
         if (BK_) { nn = idx / BK; kk = idx % BK; } else { nn = idx % BN; kk = idx / BN; }
         Bs[q][nn * PK + kk] = rb[q][i];
@@ -505,12 +506,12 @@ __device__ __forceinline__ void gemm_bf16_tile(const GemmDims& d, const Opnd<uns
   __syncthreads();  // LDS is reused by the next tile of a persistent caller
 }
 
-template <int BM, int BN, bool AK, bool BK_, bool SPLIT, bool HAS_S, bool VEC, class Epi, int BKT = 32>
-__global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmDims d, Opnd<unsigned short> A, Opnd<unsigned short> B,
+template <int BM, int BN, bool AK, bool BK_, bool SPLIT, bool HAS_S, bool VEC, class Epi, int BKT = 32, int NT = 256>
+__global__ __launch_bounds__(NT) void gemm_bf16_kernel(GemmDims d, Opnd<unsigned short> A, Opnd<unsigned short> B,
                                                         const float* __restrict__ s, long s_bs, Epi epi) {
   int tm, tn;
   tile_of_block(d, tm, tn);
-  gemm_bf16_tile<BM, BN, AK, BK_, SPLIT, HAS_S, VEC, Epi, BKT>(d, A, B, s, s_bs, epi, tm, tn, blockIdx.z, blockIdx.y);
+  gemm_bf16_tile<BM, BN, AK, BK_, SPLIT, HAS_S, VEC, Epi, BKT, NT>(d, A, B, s, s_bs, epi, tm, tn, blockIdx.z, blockIdx.y);
 }
 
 // ------------------------------------------------------------------ host launch helpers
@@ -582,6 +583,23 @@ inline hipError_t launch_gemm_bf16_big(hipStream_t st, const GemmDims& d, Opnd<u
   } else {
     if (s) hipLaunchKernelGGL((gemm_bf16_kernel<256, 256, AK, BK_, false, true, true, Epi, 32>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
     else hipLaunchKernelGGL((gemm_bf16_kernel<256, 256, AK, BK_, false, false, true, Epi, 32>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+  }
+  return hipGetLastError();
+}
+
+// 256 x 128 block tile with 512 threads (8 waves, wave tile 64 x 64), K step 64, vector loads:
+// a quarter fewer operand bytes per MFMA than 128 x 128 at the same per-wave register budget.
+template <bool AK, bool BK_, class Epi>
+inline hipError_t launch_gemm_bf16_tall(hipStream_t st, const GemmDims& d, Opnd<unsigned short> A,
+                                        Opnd<unsigned short> B, const float* s, long s_bs, bool split, Epi epi) {
+  static_assert(AK && BK_, "K-contiguous operands only");
+  dim3 grid(gemm_tiles(d), d.ksplit, d.batch);
+  if (split) {
+    if (s) hipLaunchKernelGGL((gemm_bf16_kernel<256, 128, AK, BK_, true, true, true, Epi, 64, 512>), grid, dim3(512), 0, st, d, A, B, s, s_bs, epi);
+    else hipLaunchKernelGGL((gemm_bf16_kernel<256, 128, AK, BK_, true, false, true, Epi, 64, 512>), grid, dim3(512), 0, st, d, A, B, s, s_bs, epi);
+  } else {
+    if (s) hipLaunchKernelGGL((gemm_bf16_kernel<256, 128, AK, BK_, false, true, true, Epi, 64, 512>), grid, dim3(512), 0, st, d, A, B, s, s_bs, epi);
+    else hipLaunchKernelGGL((gemm_bf16_kernel<256, 128, AK, BK_, false, false, true, Epi, 64, 512>), grid, dim3(512), 0, st, d, A, B, s, s_bs, epi);
   }
   return hipGetLastError();
 }

@@ -23,7 +23,7 @@ inline bool sym_eig(int n, const double* A, int lda, double* w, double* Z) {
   // Q accumulates the Householder reflections
   std::vector<double> q((size_t)n * n, 0.0);
   for (int i = 0; i < n; ++i) q[(size_t)i * n + i] = 1.0;
-  std::vector<double> v(n), p(n), wv(n);
+  std::vector<double> v(n), p(n), wv(n), tq(n);
   auto at = [&](int i, int j) -> double& { return a[(size_t)j * n + i]; };
   for (int k = 0; k + 2 < n; ++k) {
     // x = a[k+1:n, k]
@@ -41,12 +41,15 @@ inline bool sym_eig(int n, const double* A, int lda, double* w, double* Z) {
     for (int i = k + 1; i < n; ++i) vnorm2 += v[i] * v[i];
     if (vnorm2 == 0.0) continue;
     double beta = 2.0 / vnorm2;
-    // p = beta * A v (trailing block rows/cols k..n-1 where v is non-zero from k+1)
-    for (int i = k; i < n; ++i) {
-      double s = 0.0;
-      for (int j = k + 1; j < n; ++j) s += at(i, j) * v[j];
-      p[i] = beta * s;
+    // p = beta * A v (trailing block rows/cols k..n-1 where v is non-zero from k+1), swept
+    // column by column so every access is contiguous
+    for (int i = k; i < n; ++i) p[i] = 0.0;
+    for (int j = k + 1; j < n; ++j) {
+      const double vj = v[j];
+      const double* col = &a[(size_t)j * n];
+      for (int i = k; i < n; ++i) p[i] += col[i] * vj;
     }
+    for (int i = k; i < n; ++i) p[i] *= beta;
     double pv = 0.0;
     for (int i = k + 1; i < n; ++i) pv += p[i] * v[i];
     double c = 0.5 * beta * pv;
@@ -54,12 +57,17 @@ inline bool sym_eig(int n, const double* A, int lda, double* w, double* Z) {
     // A <- A - v w^T - w v^T  (v[k] = 0)
     for (int j = k; j < n; ++j)
       for (int i = k; i < n; ++i) at(i, j) -= v[i] * wv[j] + wv[i] * v[j];
-    // Q <- Q H, H = I - beta v v^T
-    for (int i = 0; i < n; ++i) {
-      double s = 0.0;
-      for (int j = k + 1; j < n; ++j) s += q[(size_t)j * n + i] * v[j];
-      s *= beta;
-      for (int j = k + 1; j < n; ++j) q[(size_t)j * n + i] -= s * v[j];
+    // Q <- Q H, H = I - beta v v^T: t = Q v, then Q -= beta t v^T (column sweeps)
+    for (int i = 0; i < n; ++i) tq[i] = 0.0;
+    for (int j = k + 1; j < n; ++j) {
+      const double vj = v[j];
+      const double* col = &q[(size_t)j * n];
+      for (int i = 0; i < n; ++i) tq[i] += col[i] * vj;
+    }
+    for (int j = k + 1; j < n; ++j) {
+      const double f = beta * v[j];
+      double* col = &q[(size_t)j * n];
+      for (int i = 0; i < n; ++i) col[i] -= tq[i] * f;
     }
   }
   std::vector<double> d(n), e(n, 0.0);
@@ -115,6 +123,210 @@ inline bool sym_eig(int n, const double* A, int lda, double* w, double* Z) {
   for (int j = 0; j < n; ++j) {
     w[j] = d[idx[j]];
     for (int i = 0; i < n; ++i) Z[(size_t)j * n + i] = q[(size_t)idx[j] * n + i];
+  }
+  return true;
+}
+
+// The `need` lowest eigenpairs only, for the Rayleigh-Ritz step of the Davidson loop:
+// Householder tridiagonalisation with the reflectors kept (no explicit Q), all eigenvalues of
+// the tridiagonal matrix by the implicit QR iteration without vectors, eigenvectors of the
+// tridiagonal matrix for the wanted ones by inverse iteration with partial pivoting
+// (Wilkinson's tinvit scheme: Golub & Van Loan 8.2.2 / Wilkinson & Reinsch II/18), vectors
+// of close eigenvalues re-orthogonalised against each other, then the reflectors applied to
+// the `need` vectors. O(n^3 / 3 + n^2 need) instead of O(8 n^3) for the full decomposition.
+// w[need] ascending, Z (n x need column-major). Returns false if QR did not converge.
+inline bool sym_eig_lowest(int n, const double* A, int lda, int need, double* w, double* Z) {
+  if (n <= 0 || need <= 0) return true;
+  if (need > n) need = n;
+  std::vector<double> a((size_t)n * n);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) a[(size_t)j * n + i] = 0.5 * (A[(size_t)j * lda + i] + A[(size_t)i * lda + j]);
+  auto at = [&](int i, int j) -> double& { return a[(size_t)j * n + i]; };
+  std::vector<double> refl((size_t)n * n, 0.0), betas(n, 0.0);  // reflector k in column k
+  std::vector<double> p(n), wv(n);
+  for (int k = 0; k + 2 < n; ++k) {
+    double alpha = 0.0;
+    for (int i = k + 1; i < n; ++i) alpha += at(i, k) * at(i, k);
+    alpha = std::sqrt(alpha);
+    if (alpha == 0.0) continue;
+    if (at(k + 1, k) > 0) alpha = -alpha;
+    double* v = &refl[(size_t)k * n];
+    for (int i = k + 1; i < n; ++i) v[i] = at(i, k);
+    v[k + 1] -= alpha;
+    double vn = 0.0;
+    for (int i = k + 1; i < n; ++i) vn += v[i] * v[i];
+    if (vn == 0.0) continue;
+    const double beta = 2.0 / vn;
+    betas[k] = beta;
+    // p = beta * A v from the lower triangle only (A symmetric; v[k] = 0)
+    for (int i = k; i < n; ++i) p[i] = 0.0;
+    for (int j = k; j < n; ++j) {
+      const double* col = &a[(size_t)j * n];
+      const double vj = v[j];
+      double dot = 0.0;
+      p[j] += col[j] * vj;
+      for (int i = j + 1; i < n; ++i) {
+        p[i] += col[i] * vj;
+        dot += col[i] * v[i];
+      }
+      p[j] += dot;
+    }
+    for (int i = k; i < n; ++i) p[i] *= beta;
+    double pv = 0.0;
+    for (int i = k + 1; i < n; ++i) pv += p[i] * v[i];
+    const double c = 0.5 * beta * pv;
+    // v[i] = 0 for i <= k (reflector storage starts zeroed)
+    for (int i = k; i < n; ++i) wv[i] = p[i] - c * v[i];
+    for (int j = k; j < n; ++j) {  // lower triangle only
+      const double vj = v[j], wj = wv[j];
+      double* col = &a[(size_t)j * n];
+      for (int i = j; i < n; ++i) col[i] -= v[i] * wj + wv[i] * vj;
+    }
+  }
+  std::vector<double> d(n), e(n, 0.0);
+  for (int i = 0; i < n; ++i) d[i] = at(i, i);
+  for (int i = 0; i + 1 < n; ++i) e[i] = at(i + 1, i);
+  double tnorm = 0.0;
+  for (int i = 0; i < n; ++i) tnorm = std::max(tnorm, std::fabs(d[i]) + std::fabs(e[i]) + (i ? std::fabs(e[i - 1]) : 0.0));
+  if (tnorm == 0.0) tnorm = 1.0;
+
+  // eigenvalues of the tridiagonal matrix (implicit QR, Wilkinson shift, no vectors)
+  std::vector<double> dd(d), ee(e);
+  const double eps = 2.220446049250313e-16;
+  {
+    int h = n - 1, iter = 0;
+    const int maxiter = 60 * n + 100;
+    while (h > 0) {
+      if (std::fabs(ee[h - 1]) <= eps * (std::fabs(dd[h - 1]) + std::fabs(dd[h]))) {
+        ee[h - 1] = 0.0;
+        --h;
+        continue;
+      }
+      int l = h - 1;
+      while (l > 0 && std::fabs(ee[l - 1]) > eps * (std::fabs(dd[l - 1]) + std::fabs(dd[l]))) --l;
+      if (++iter > maxiter) return false;
+      double t = 0.5 * (dd[h - 1] - dd[h]);
+      double e2 = ee[h - 1] * ee[h - 1];
+      double den = t + (t >= 0 ? 1.0 : -1.0) * std::sqrt(t * t + e2);
+      double mu = dd[h] - (den != 0.0 ? e2 / den : 0.0);
+      double x = dd[l] - mu, z = ee[l];
+      for (int k = l; k < h; ++k) {
+        double r = std::hypot(x, z);
+        double cc = 1.0, ss = 0.0;
+        if (r != 0.0) { cc = x / r; ss = z / r; }
+        if (k > l) ee[k - 1] = r;
+        double a0 = dd[k], b0 = ee[k], c0 = dd[k + 1];
+        dd[k] = cc * cc * a0 + 2.0 * cc * ss * b0 + ss * ss * c0;
+        dd[k + 1] = ss * ss * a0 - 2.0 * cc * ss * b0 + cc * cc * c0;
+        ee[k] = cc * ss * (c0 - a0) + (cc * cc - ss * ss) * b0;
+        if (k + 1 < h) {
+          z = ss * ee[k + 1];
+          ee[k + 1] = cc * ee[k + 1];
+        }
+        x = ee[k];
+      }
+    }
+  }
+  std::sort(dd.begin(), dd.end());
+
+  // inverse iteration on T - lambda I (tridiagonal LU with partial pivoting)
+  const double eps3 = eps * tnorm;                  // replacement for a zero pivot
+  const double ortol = 1e-3 * tnorm;                // eigenvalues closer than this form a cluster
+  const double growth = 1.0 / (std::sqrt((double)n) * 1e3 * eps);
+  std::vector<double> r1(n), r2(n), r3(n), mult(n), y(n), b(n);
+  std::vector<char> swp(n);
+  std::vector<double> Y((size_t)n * need);
+  unsigned long long seed = 0x9E3779B97F4A7C15ull;
+  double lam_prev = 0.0;
+  int cluster_start = 0;
+  for (int j = 0; j < need; ++j) {
+    double lam = dd[j];
+    if (j > 0) {
+      if (lam - dd[j - 1] > ortol) cluster_start = j;
+      if (lam - lam_prev < 10.0 * eps3) lam = lam_prev + 10.0 * eps3;  // separate equal shifts
+    }
+    lam_prev = lam;
+    // factor T - lam I = P L U (U with two superdiagonals)
+    double u = d[0] - lam, v = n > 1 ? e[0] : 0.0;
+    for (int i = 1; i < n; ++i) {
+      const double sub = e[i - 1], di = d[i] - lam, ei = i + 1 < n ? e[i] : 0.0;
+      if (std::fabs(sub) > std::fabs(u)) {  // strict: sub != 0 here
+        const double xu = u / sub;
+        mult[i] = xu;
+        swp[i] = 1;
+        r1[i - 1] = sub;
+        r2[i - 1] = di;
+        r3[i - 1] = ei;
+        u = v - xu * di;
+        v = -xu * ei;
+      } else {
+        const double xu = sub / (u != 0.0 ? u : eps3);
+        mult[i] = xu;
+        swp[i] = 0;
+        r1[i - 1] = u != 0.0 ? u : eps3;
+        r2[i - 1] = v;
+        r3[i - 1] = 0.0;
+        u = di - xu * v;
+        v = ei;
+      }
+    }
+    r1[n - 1] = u != 0.0 ? u : eps3;
+    r2[n - 1] = 0.0;
+    r3[n - 1] = 0.0;
+    for (int i = 0; i + 1 < n; ++i)
+      if (r1[i] == 0.0) r1[i] = eps3;
+    // start vector: pseudo-random, deterministic
+    for (int i = 0; i < n; ++i) {
+      seed = seed * 6364136223846793005ull + 1442695040888963407ull;
+      b[i] = ((double)(seed >> 11) * (1.0 / 9007199254740992.0)) - 0.5;
+    }
+    double* yj = &Y[(size_t)j * n];
+    for (int it = 0; it < 6; ++it) {
+      if (it > 0) {  // forward elimination of the right-hand side with the recorded pivots
+        for (int i = 1; i < n; ++i) {
+          if (swp[i]) std::swap(b[i - 1], b[i]);
+          b[i] -= mult[i] * b[i - 1];
+        }
+      }
+      for (int i = n - 1; i >= 0; --i) {
+        double s = b[i];
+        if (i + 1 < n) s -= r2[i] * y[i + 1];
+        if (i + 2 < n) s -= r3[i] * y[i + 2];
+        y[i] = s / r1[i];
+      }
+      // orthogonalise against the earlier vectors of this cluster (twice)
+      for (int rep = 0; rep < 2; ++rep)
+        for (int c = cluster_start; c < j; ++c) {
+          const double* yc = &Y[(size_t)c * n];
+          double s = 0.0;
+          for (int i = 0; i < n; ++i) s += yc[i] * y[i];
+          for (int i = 0; i < n; ++i) y[i] -= s * yc[i];
+        }
+      double nrm = 0.0;
+      for (int i = 0; i < n; ++i) nrm += y[i] * y[i];
+      nrm = std::sqrt(nrm);
+      if (nrm == 0.0) {  // restart from another vector
+        for (int i = 0; i < n; ++i) b[i] = (i % (j + 2) == 0) ? 1.0 : 0.0;
+        continue;
+      }
+      for (int i = 0; i < n; ++i) b[i] = y[i] / nrm;
+      if (nrm >= growth && it >= 1) break;
+    }
+    for (int i = 0; i < n; ++i) yj[i] = b[i];
+    w[j] = dd[j];
+  }
+  // back-transform: Z = H_0 H_1 ... H_{n-3} Y
+  for (int j = 0; j < need; ++j) {
+    double* z = &Y[(size_t)j * n];
+    for (int k = n - 3; k >= 0; --k) {
+      if (betas[k] == 0.0) continue;
+      const double* v = &refl[(size_t)k * n];
+      double s = 0.0;
+      for (int i = k + 1; i < n; ++i) s += v[i] * z[i];
+      s *= betas[k];
+      for (int i = k + 1; i < n; ++i) z[i] -= s * v[i];
+    }
+    for (int i = 0; i < n; ++i) Z[(size_t)j * n + i] = z[i];
   }
   return true;
 }

@@ -179,6 +179,7 @@ inline int bf16_kernel_choice() {
   if (e && std::strcmp(e, "staged32") == 0) return 0;
   if (e && std::strcmp(e, "pipe") == 0) return 2;
   if (e && std::strcmp(e, "big") == 0) return 3;
+  if (e && std::strcmp(e, "tall") == 0) return 4;
   return 1;
 }
 
@@ -193,7 +194,21 @@ void gemm16(hipStream_t st, int M, int N, int K, int batch, int ksplit, Opnd<uns
       XG_HIP((launch_gemm_bf16_big<AK, BK_>(st, d, A, B, s, s_bs, split, epi)));
       return;
     }
-    if ((choice == 1 || choice == 3) && bf16_vec(A, B, K, s_bs)) {
+    if (choice == 4 && bf16_vec(A, B, K, s_bs) && (long)M * N >= 256L * 128 * 512) {
+      GemmDims d = make_dims<256, 128>(M, N, std::max(K, 0), batch, ksplit, 64, lower);
+      if (!lower) d.group = tile_group(d);
+      XG_HIP((launch_gemm_bf16_tall<AK, BK_>(st, d, A, B, s, s_bs, split, epi)));
+      return;
+    }
+    if ((choice == 1 || choice == 3 || choice == 4) && bf16_vec(A, B, K, s_bs)) {
+      const long wide_blocks = (long)((M + 127) / 128) * ((N + 127) / 128) * batch * std::max(ksplit, 1);
+      if (!lower && N <= 128 && wide_blocks < 512) {
+        // narrow outputs with few tiles (Hd stage 2: 900 x 100 per vector): 128 x 64 tiles
+        // double the workgroup count so every CU gets work
+        GemmDims d = make_dims<128, 64>(M, N, std::max(K, 0), batch, ksplit, 64, false);
+        XG_HIP((launch_gemm_bf16<128, 64, AK, BK_>(st, d, A, B, s, s_bs, split, epi, true)));
+        return;
+      }
       GemmDims d = make_dims<128, 128>(M, N, std::max(K, 0), batch, ksplit, 64, lower);
       if (!lower) d.group = tile_group(d);
       XG_HIP((launch_gemm_bf16<128, 128, AK, BK_>(st, d, A, B, s, s_bs, split, epi, true)));
