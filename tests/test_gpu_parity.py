@@ -203,6 +203,70 @@ def test_bf16_kernels_agree(ctx, c, shape, monkeypatch):
     assert relerr(ys["staged64"], ys["staged32"]) < 2e-5
 
 
+@pytest.mark.parametrize("vt,ct,P", [(1, 1, 1), (1, 17, 3), (9, 1, 5), (3, 8, 1)])
+@pytest.mark.parametrize("c", [(1, 2, 1, 0), (0, 2, 0, 1), (1, 0, 1, 0)])
+def test_bse_operator_degenerate_shapes(ctx, c, vt, ct, P):
+    """Single valence / conduction level and single auxiliary function: every padded extent is a
+    ragged tile edge. FP64 against the oracle, BF16x3 against FP64; k = 1 and k = 0 columns."""
+    M, eps, Hqp, homo, rpamin, vmin, cmax = random_bse_case(30 + vt + ct + P, vt=vt, ct=ct, P=P, offdiag=True)
+    t = dev_mmn(ctx, M)
+    op = px.BSE_OPERATOR(*c, eps, t, Hqp, precisions=("fp64", "bf16x3")).configure(
+        px.BSEOperator_Options(homo, rpamin, 0, vmin, cmax))
+    ref_op = X.BSEOperatorOracle(*c, eps, M, Hqp, homo, rpamin, 0, vmin, cmax)
+    assert op.size == vt * ct
+    for k in (1, 3):
+        Xv = np.random.default_rng(k).normal(size=(op.size, k))
+        ref = ref_op.matmul(Xv)
+        assert relerr(op.matmul(Xv), ref) < 1e-13
+        assert relerr(op.matmul(Xv, "bf16x3"), ref) < 2e-5
+    assert relerr(op.diagonal(), ref_op.diagonal()) < 1e-13
+    assert op.matmul(np.zeros((op.size, 0))).shape == (op.size, 0)
+
+
+def test_argument_errors_raise(ctx):
+    """Shape and option errors come back as exceptions (the reference throws std::runtime_error)
+    and leave the context usable."""
+    M, eps, Hqp, homo, rpamin, vmin, cmax = random_bse_case(31, vt=3, ct=5, P=4)
+    t = dev_mmn(ctx, M)
+    opt = px.BSEOperator_Options(homo, rpamin, 0, vmin, cmax)
+    with pytest.raises(ValueError):
+        px.BSE_OPERATOR(0, 0, 1, 1, eps, t, Hqp)
+    op = px.BSE_OPERATOR(1, 2, 1, 0, eps, t, Hqp).configure(opt)  # FP64 only
+    with pytest.raises(px.XtpGpuError):
+        op.matmul(np.ones((op.size, 2)), "bf16x3")
+    with pytest.raises(px.XtpGpuError):
+        px._check(px.lib().xtpgpu_bse_matmul(op.h, 0, px._dptr(np.ones(op.size)), op.size - 1, 1,
+                                             px._dptr(np.ones(op.size)), op.size))
+    ds = px.DavidsonSolver()
+    with pytest.raises(px.XtpGpuError):
+        ds.solve(op, op.size)  # 2*neigen exceeds the operator size
+    Xv = np.random.default_rng(0).normal(size=(op.size, 2))
+    assert relerr(op.matmul(Xv), X.BSEOperatorOracle(1, 2, 1, 0, eps, M, Hqp, homo, rpamin, 0, vmin, cmax).matmul(Xv)) < 1e-13
+
+
+def test_device_copy_kernel(ctx):
+    rng = np.random.default_rng(5)
+    a = rng.normal(size=4096 * 3)
+    src, dst = px.DeviceBuffer(ctx, a.nbytes), px.DeviceBuffer(ctx, a.nbytes)
+    src.upload(a)
+    px._check(px.lib().xtpgpu_copy_device(ctx.h, dst.ptr, src.ptr, a.nbytes))
+    assert np.array_equal(dst.download(a.shape), a)
+    with pytest.raises(px.XtpGpuError):
+        px._check(px.lib().xtpgpu_copy_device(ctx.h, dst.ptr, src.ptr, 8))
+
+
+def test_davidson_single_root(ctx):
+    M, eps, Hqp, homo, rpamin, vmin, cmax = random_bse_case(32, vt=4, ct=25, P=30, scale=0.05)
+    t = dev_mmn(ctx, M)
+    op = px.SingletOperator_TDA(eps, t, Hqp).configure(px.BSEOperator_Options(homo, rpamin, 0, vmin, cmax))
+    w = np.linalg.eigvalsh(X.bse_dense(1, 2, 1, 0, eps, M, Hqp, homo, rpamin, vmin, cmax))
+    ds = px.DavidsonSolver()
+    ds.set_tolerance(1e-9)
+    ev, vec = ds.solve(op, 1)
+    assert ds.converged()
+    assert abs(ev[0] - w[0]) < 1e-10
+
+
 def test_davidson_fp64_matches_dense(ctx):
     M, eps, Hqp, homo, rpamin, vmin, cmax = random_bse_case(20, vt=8, ct=40, P=60, scale=0.05)
     t = dev_mmn(ctx, M)

@@ -85,6 +85,19 @@ struct EpiPartial {  // split-K partials: part[(ks*batch + b)*M*N + n*M + m]
   __device__ void operator()(int b, int ks, int m, int n, V v) const { put(row(b, ks, m), col(b, n), v); }
 };
 
+struct EpiPart32 {  // FP32 split-K partials of a batched product: part[((ks*batch + b)*N + n)*M + m]
+  float* part;
+  long M, N, batch;
+  static constexpr bool kSeparable = true;
+  typedef long Row;
+  __device__ long row(int b, int ks, int m) const { return ((long)ks * batch + b) * N * M + m; }
+  __device__ long col(int, int n) const { return (long)n * M; }
+  template <typename V>
+  __device__ void put(long r, long c, V v) const { part[r + c] = (float)v; }
+  template <typename V>
+  __device__ void operator()(int b, int ks, int m, int n, V v) const { put(row(b, ks, m), col(b, n), v); }
+};
+
 struct EpiEpsSym {  // eps[b](m,n) = eps[b](n,m) = v + delta_mn, written once from m >= n
   double* out;
   long P;
@@ -201,14 +214,6 @@ void gemm16(hipStream_t st, int M, int N, int K, int batch, int ksplit, Opnd<uns
       return;
     }
     if ((choice == 1 || choice == 3 || choice == 4) && bf16_vec(A, B, K, s_bs)) {
-      const long wide_blocks = (long)((M + 127) / 128) * ((N + 127) / 128) * batch * std::max(ksplit, 1);
-      if (!lower && N <= 128 && wide_blocks < 512) {
-        // narrow outputs with few tiles (Hd stage 2: 900 x 100 per vector): 128 x 64 tiles
-        // double the workgroup count so every CU gets work
-        GemmDims d = make_dims<128, 64>(M, N, std::max(K, 0), batch, ksplit, 64, false);
-        XG_HIP((launch_gemm_bf16<128, 64, AK, BK_>(st, d, A, B, s, s_bs, split, epi, true)));
-        return;
-      }
       GemmDims d = make_dims<128, 128>(M, N, std::max(K, 0), batch, ksplit, 64, lower);
       if (!lower) d.group = tile_group(d);
       XG_HIP((launch_gemm_bf16<128, 128, AK, BK_>(st, d, A, B, s, s_bs, split, epi, true)));
@@ -867,11 +872,33 @@ void bse_matmul_impl(xtpgpu_bseop op, int precision, const double* X, long ldx, 
                              Opnd<unsigned short>{Xh, Xl, ctp, 0}, nullptr, 0, split, false, e1);
         }
         ProfScope prof(ctx, "hd_stage2", f2, b2);
-        gemm16<true, true>(st, (int)ct, (int)vt, (int)(np * vtp), (int)k, 1,
-                           Opnd<unsigned short>{Sh, Sl, k * np * vtp, np * vtp},
-                           Opnd<unsigned short>{op->Bvv_h.as<unsigned short>() + p0 * vtp,
-                                                split ? op->Bvv_l.as<unsigned short>() + p0 * vtp : nullptr, P * vtp, 0},
-                           nullptr, 0, split, false, EpiStore{Y, 1, ct, ldy, alpha, 1.0});
+        const Opnd<unsigned short> s2a{Sh, Sl, k * np * vtp, np * vtp};
+        const Opnd<unsigned short> s2b{op->Bvv_h.as<unsigned short>() + p0 * vtp,
+                                       split ? op->Bvv_l.as<unsigned short>() + p0 * vtp : nullptr, P * vtp, 0};
+        // few output tiles (ct x vt per vector) over a long K = (P, v'): split K so the grid
+        // covers the chip, FP32 partials, then a fixed-order reduction into Y
+        const long tiles2 = ((ct + 127) / 128) * ((vt + 127) / 128) * k;
+        const int ks2 = (int)std::max<long>(1, std::min<long>(4, std::min<long>(512 / std::max<long>(1, tiles2),
+                                                                               (np * vtp) / 2048)));
+        if (ks2 > 1 && bf16_vec(s2a, s2b, (int)(np * vtp), 0)) {
+          // launched directly so the partial count is exactly dd.ksplit
+          const GemmDims dd = make_dims<128, 128>((int)ct, (int)vt, (int)(np * vtp), (int)k, ks2, 64, false);
+          op->part.ensure((size_t)dd.ksplit * k * ct * vt * 4);
+          float* pt = op->part.as<float>();
+          XG_HIP((launch_gemm_bf16<128, 128, true, true>(st, dd, s2a, s2b, nullptr, 0, split, EpiPart32{pt, ct, vt, k},
+                                                         true)));
+          const long nout = k * ct * vt, nsp = dd.ksplit, ctl = ct, vtl = vt;
+          launch_map(st, (size_t)nout, [=] __device__(size_t i) {
+            double acc = 0.0;
+            for (long q = 0; q < nsp; ++q) acc += (double)pt[q * nout + (long)i];
+            const long b = (long)i / (ctl * vtl), r = (long)i - b * ctl * vtl;
+            const long n = r / ctl, m = r - n * ctl;
+            Y[b * ldy + n * ctl + m] += alpha * acc;
+          });
+        } else {
+          gemm16<true, true>(st, (int)ct, (int)vt, (int)(np * vtp), (int)k, 1, s2a, s2b, nullptr, 0, split, false,
+                             EpiStore{Y, 1, ct, ldy, alpha, 1.0});
+        }
       }
     }
   }
@@ -1140,6 +1167,9 @@ extern "C" int xtpgpu_bse_matmul(xtpgpu_bseop op, int precision, const double* X
   return guarded([&] {
     XG_REQUIRE(op && X && Y, XTPGPU_ERR_ARG, "null argument");
     XG_REQUIRE(ldx >= op->N && ldy >= op->N && k >= 0, XTPGPU_ERR_SHAPE, "xtpgpu_bse_matmul: bad leading dimension");
+    XG_REQUIRE(precision == XTPGPU_FP64 || ((op->mask >> precision) & 1u), XTPGPU_ERR_ARG,
+               "bse matmul: operator was created without this precision");
+    if (k == 0) return;
     XG_HIP(hipSetDevice(op->ctx->device));
     hipStream_t st = op->ctx->stream;
     const size_t n = (size_t)op->N * k;
