@@ -503,7 +503,16 @@ def main():
                 "achieved": ach / 1e12,
                 "peak": PEAK["bf16"] / 1e12, "unit": "TFLOP/s", "frac": ach / PEAK["bf16"], "traffic": traffic,
                 "traffic_launch": HD_STAGE1_TRAFFIC["launch"] if traffic else None,
-                "avg_launch_ms": kt * 1e3, "algorithmic_flops_per_launch": tm["flops"] / max(1, tm["launches"])}
+                "avg_launch_ms": kt * 1e3, "algorithmic_flops_per_launch": tm["flops"] / max(1, tm["launches"]),
+                # the split form issues three BF16 MFMAs per algorithmic product
+                "mfma_issued_tflops": (3.0 if args.mixed == "bf16x3" else 1.0) * ach / 1e12,
+                "mfma_issued_frac": (3.0 if args.mixed == "bf16x3" else 1.0) * ach / PEAK["bf16"]}
+    if "fp64" in res:
+        t64 = res["fp64"]["hd_stage1"]
+        a64 = t64["flops"] / max(t64["ms"] / 1e3, 1e-30)
+        roofline["fp64_form"] = {"kernel": "hd_stage1_fp64 (gemm_f64_kernel<128,128,..,EpiStage1>)",
+                                 "achieved": a64 / 1e12, "peak": PEAK["fp64"] / 1e12, "frac": a64 / PEAK["fp64"],
+                                 "avg_launch_ms": t64["ms"] / max(1, t64["launches"])}
 
     cpu = None
     if not args.no_cpu:
