@@ -360,6 +360,9 @@ def main():
                     help="amplitude of sigma_P = sigma0 exp(-3P/n_aux); 0.3 is the literal config value, "
                          "on which the reference Davidson does not converge (exchange ~100x physical)")
     ap.add_argument("--iter-max", type=int, default=300)
+    ap.add_argument("--ladder", default="bf16", choices=["bf16", "none"],
+                    help="mixed solve: iterate in plain BF16 first, then in --mixed (precision ladder)")
+    ap.add_argument("--switch-tol", type=float, default=1e-5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="single", choices=["single", "segments", "matvec", "adjoint"])
     ap.add_argument("--adj-roots", type=int, default=5)
@@ -443,7 +446,8 @@ def main():
     Mmn.MultiplyRightWithAuxMatrix(U)
     epsinv = np.where(lam > 1e-8, 1.0 / lam, 0.0)
     Hqp = np.diag(e)
-    op = px.SingletOperator_TDA(epsinv, Mmn, Hqp, precisions=("fp64", args.mixed)).configure(
+    precs = ("fp64", args.mixed) + (("bf16",) if args.ladder == "bf16" else ())
+    op = px.SingletOperator_TDA(epsinv, Mmn, Hqp, precisions=precs).configure(
         px.BSEOperator_Options(o - 1, 0, 0, 0, n - 1))
 
     def solve(prec):
@@ -454,7 +458,8 @@ def main():
         ds.set_max_search_space(10 * args.roots)
         ds.set_iter_max(args.iter_max)
         if prec != "fp64":
-            ds.set_precision(prec, refine_tolerance=args.tol, refine_max=4)
+            ds.set_precision(prec, refine_tolerance=args.tol, refine_max=4,
+                             start=args.ladder if args.ladder != "none" else None, switch_tolerance=args.switch_tol)
         ctx.reset_timers()
         t0 = time.perf_counter()
         ev, _ = ds.solve(op, args.roots, want_vectors=False)
@@ -526,11 +531,12 @@ def main():
         "metric": "BSE-TDA 20-root solve s & RPA eps TFLOP/s: mixed vs FP64 GPU at <=1 meV",
         "value": t_mixed, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": (t_mixed + eps_res[args.mixed]["wall_s"]) * 1e3,
-        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3/fp32+fp64",
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": ("bf16,then " if args.ladder == "bf16" else "") + f"{args.mixed}/fp32+fp64",
         "data": "synthetic",
         "config": {"workload": f"synthetic single segment n_occ={o} n_virt={v} n_aux={P}", "n_occ": o, "n_virt": v, "n_aux": P,
                    "sigma0": args.sigma0,
-                   "n_freq": 12, "roots": args.roots, "tol": args.tol, "mixed": args.mixed},
+                   "n_freq": 12, "roots": args.roots, "tol": args.tol, "mixed": args.mixed,
+                   "ladder": args.ladder, "switch_tol": args.switch_tol},
         "bse_tda_solve_s": {"mixed": t_mixed, "fp64": res.get("fp64", {}).get("wall_s"),
                             "speedup": (res["fp64"]["wall_s"] / t_mixed) if "fp64" in res else None},
         "rpa_eps_tflops": {"mixed": eps_res[args.mixed]["tflops"], "fp64": eps_res["fp64"]["tflops"],

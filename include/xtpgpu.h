@@ -225,6 +225,13 @@ typedef struct {
    * the Ritz vectors plus DPR corrections, until the FP64 residual norm < refine_tolerance. */
   double refine_tolerance;
   int refine_max;
+  /* Precision ladder (SYMM solve, precision != FP64): iterate first with precision_start
+   * (e.g. BF16) until every wanted residual norm is below switch_tolerance or has stopped
+   * decreasing for 20 iterations, then restart on the leading Ritz vectors (A*V recomputed)
+   * and continue with `precision`. precision_start < 0 (or equal to precision) disables it.
+   * No counterpart in the reference (whose Davidson is FP64 only). */
+  int precision_start;
+  double switch_tolerance;
 } xtpgpu_davidson_opts;
 
 typedef struct {

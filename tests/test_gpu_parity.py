@@ -305,6 +305,28 @@ def test_davidson_mixed_refined_matches_dense(ctx, prec):
     assert np.abs(ev - w[:5]).max() < etol
 
 
+def test_davidson_precision_ladder_matches_dense(ctx):
+    """BF16 iterations to 1e-4, restart on the Ritz vectors with BF16x3, FP64 refinement."""
+    M, eps, Hqp, homo, rpamin, vmin, cmax = random_bse_case(21, vt=8, ct=40, P=60, scale=0.05)
+    t = dev_mmn(ctx, M)
+    op = px.SingletOperator_TDA(eps, t, Hqp, precisions=("fp64", "bf16", "bf16x3")).configure(
+        px.BSEOperator_Options(homo, rpamin, 0, vmin, cmax))
+    w = np.linalg.eigvalsh(X.bse_dense(1, 2, 1, 0, eps, M, Hqp, homo, rpamin, vmin, cmax))
+    ds = px.DavidsonSolver()
+    ds.set_max_search_space(50)
+    ds.set_tolerance(1e-6)
+    ds.set_iter_max(100)
+    ds.set_precision("bf16x3", refine_tolerance=1e-7, refine_max=12, start="bf16", switch_tolerance=1e-4)
+    ev, vec = ds.solve(op, 5)
+    assert ds.converged(), (ds.info.max_residual, ds.info.refine_rounds)
+    assert np.abs(ev - w[:5]).max() < 1e-10
+    # an operator built without the start precision is refused
+    op2 = px.SingletOperator_TDA(eps, t, Hqp, precisions=("fp64", "bf16x3")).configure(
+        px.BSEOperator_Options(homo, rpamin, 0, vmin, cmax))
+    with pytest.raises(px.XtpGpuError):
+        ds.solve(op2, 5)
+
+
 def test_davidson_forced_failure(ctx):  # test_davidson.cc:107-125
     M, eps, Hqp, homo, rpamin, vmin, cmax = random_bse_case(22, vt=5, ct=30, P=40)
     t = dev_mmn(ctx, M)

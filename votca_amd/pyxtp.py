@@ -37,7 +37,8 @@ class XtpGpuError(RuntimeError):
 class DavidsonOpts(C.Structure):
     _fields_ = [("tolerance", C.c_double), ("correction", C.c_int), ("size_update", C.c_int),
                 ("iter_max", C.c_int), ("max_search_space", C.c_int), ("precision", C.c_int),
-                ("refine_tolerance", C.c_double), ("refine_max", C.c_int)]
+                ("refine_tolerance", C.c_double), ("refine_max", C.c_int), ("precision_start", C.c_int),
+                ("switch_tolerance", C.c_double)]
 
 
 class DavidsonInfo(C.Structure):
@@ -482,7 +483,7 @@ class DavidsonSolver:
     _TOL = {"loose": 1e-3, "normal": 1e-4, "strict": 1e-5, "lapack": 1e-9}
 
     def __init__(self):
-        self.opts = DavidsonOpts(1e-4, 0, 1, 50, 0, FP64, 1e-4, 0)
+        self.opts = DavidsonOpts(1e-4, 0, 1, 50, 0, FP64, 1e-4, 0, -1, 0.0)
         self.info = DavidsonInfo()
 
     def set_iter_max(self, n):
@@ -509,10 +510,14 @@ class DavidsonSolver:
             raise RuntimeError(s + " is not a valid Davidson update")
         self.opts.size_update = m[s]
 
-    def set_precision(self, precision, refine_tolerance=None, refine_max=4):
+    def set_precision(self, precision, refine_tolerance=None, refine_max=4, start=None, switch_tolerance=1e-5):
+        """Operator precision of the iterations; FP64 refinement rounds for the mixed forms; an
+        optional cheaper `start` precision used until the residuals reach `switch_tolerance`."""
         self.opts.precision = _PREC[precision]
         self.opts.refine_max = int(refine_max) if _PREC[precision] != FP64 else 0
         self.opts.refine_tolerance = float(refine_tolerance if refine_tolerance is not None else self.opts.tolerance)
+        self.opts.precision_start = _PREC[start] if start is not None else -1
+        self.opts.switch_tolerance = float(switch_tolerance)
 
     def solve(self, op: BSE_OPERATOR, neigen: int, want_vectors=True):
         ev = np.zeros(neigen)
