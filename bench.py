@@ -363,6 +363,7 @@ def main():
     ap.add_argument("--ladder", default="bf16", choices=["bf16", "none"],
                     help="mixed solve: iterate in plain BF16 first, then in --mixed (precision ladder)")
     ap.add_argument("--switch-tol", type=float, default=1e-5)
+    ap.add_argument("--refine-max", type=int, default=8, help="FP64 refinement rounds of the mixed solve")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="single", choices=["single", "segments", "matvec", "adjoint"])
     ap.add_argument("--adj-roots", type=int, default=5)
@@ -458,14 +459,19 @@ def main():
         ds.set_max_search_space(10 * args.roots)
         ds.set_iter_max(args.iter_max)
         if prec != "fp64":
-            ds.set_precision(prec, refine_tolerance=args.tol, refine_max=4,
+            ds.set_precision(prec, refine_tolerance=args.tol, refine_max=args.refine_max,
                              start=args.ladder if args.ladder != "none" else None, switch_tolerance=args.switch_tol)
         ctx.reset_timers()
         t0 = time.perf_counter()
         ev, _ = ds.solve(op, args.roots, want_vectors=False)
         wall = time.perf_counter() - t0
         info = ds.info
-        tm = ctx.timer("hd_stage1_fp64" if prec == "fp64" else "hd_stage1")
+        if prec == "fp64":
+            tm = {"fp64": ctx.timer("hd_stage1_fp64")}
+        else:
+            tm = {prec: ctx.timer("hd_stage1" if prec == "bf16x3" else "hd_stage1_bf16")}
+            if args.ladder == "bf16" and prec != "bf16":
+                tm["bf16"] = ctx.timer("hd_stage1_bf16")
         return ev, wall, info, tm
 
     res = {}
@@ -497,23 +503,35 @@ def main():
         res["fp64"]["omega_eV"] = [float(x) for x in ev64 * HRT2EV]
         log(f"[bench] max |dOmega| mixed vs fp64: {max_err_mev:.3e} meV")
 
-    tm = res["mixed"]["hd_stage1"]
+    # dominant kernel of the mixed solve: the Hd stage-1 product of the precision that took the
+    # most time (the split form issues three BF16 MFMAs per algorithmic product, plain BF16 one)
+    stage1 = res["mixed"]["hd_stage1"]
+    dom = max(stage1, key=lambda k: stage1[k]["ms"])
+    tm = stage1[dom]
     kt = tm["ms"] / 1e3 / max(1, tm["launches"])
     ach = tm["flops"] / max(tm["ms"] / 1e3, 1e-30)
+    issue = 3.0 if dom == "bf16x3" else 1.0
     kname = {"staged32": "gemm_bf16_kernel<128,128,..,EpiStage1,32>", "pipe": "gemm_bf16x3_pipe_kernel<..,EpiStage1>",
-             "big": "gemm_bf16_kernel<256,256,..,EpiStage1,32>"}.get(os.environ.get("XTPGPU_BF16_KERNEL", ""),
-                                                                    "gemm_bf16_kernel<128,128,..,EpiStage1,64>")
-    traffic = HD_STAGE1_TRAFFIC["read"] + HD_STAGE1_TRAFFIC["write"] if kname.endswith(",64>") else None
-    roofline = {"bound": "mfma", "kernel": f"hd_stage1 ({kname})",
+             "big": "gemm_bf16_kernel<256,256,..,EpiStage1,32>", "tall": "gemm_bf16_kernel<256,128,..,EpiStage1,64,512>"
+             }.get(os.environ.get("XTPGPU_BF16_KERNEL", ""), "gemm_bf16_kernel<128,128,..,EpiStage1,64>")
+    kname = kname.replace("<", "<" + ("split," if dom == "bf16x3" else "plain,"), 1)
+    traffic = (HD_STAGE1_TRAFFIC["read"] + HD_STAGE1_TRAFFIC["write"]
+               if kname.endswith(",64>") and dom == "bf16x3" else None)
+    roofline = {"bound": "mfma", "kernel": f"hd_stage1 {dom} ({kname})",
                 "achieved": ach / 1e12,
                 "peak": PEAK["bf16"] / 1e12, "unit": "TFLOP/s", "frac": ach / PEAK["bf16"], "traffic": traffic,
                 "traffic_launch": HD_STAGE1_TRAFFIC["launch"] if traffic else None,
                 "avg_launch_ms": kt * 1e3, "algorithmic_flops_per_launch": tm["flops"] / max(1, tm["launches"]),
-                # the split form issues three BF16 MFMAs per algorithmic product
-                "mfma_issued_tflops": (3.0 if args.mixed == "bf16x3" else 1.0) * ach / 1e12,
-                "mfma_issued_frac": (3.0 if args.mixed == "bf16x3" else 1.0) * ach / PEAK["bf16"]}
+                "mfma_issued_tflops": issue * ach / 1e12, "mfma_issued_frac": issue * ach / PEAK["bf16"]}
+    for other, t in stage1.items():
+        if other != dom:
+            a2 = t["flops"] / max(t["ms"] / 1e3, 1e-30)
+            roofline[f"{other}_launches"] = {"achieved": a2 / 1e12, "frac": a2 / PEAK["bf16"],
+                                             "mfma_issued_frac": (3.0 if other == "bf16x3" else 1.0) * a2 / PEAK["bf16"],
+                                             "avg_launch_ms": t["ms"] / max(1, t["launches"]), "launches": t["launches"],
+                                             "total_ms": t["ms"]}
     if "fp64" in res:
-        t64 = res["fp64"]["hd_stage1"]
+        t64 = res["fp64"]["hd_stage1"]["fp64"]
         a64 = t64["flops"] / max(t64["ms"] / 1e3, 1e-30)
         roofline["fp64_form"] = {"kernel": "hd_stage1_fp64 (gemm_f64_kernel<128,128,..,EpiStage1>)",
                                  "achieved": a64 / 1e12, "peak": PEAK["fp64"] / 1e12, "frac": a64 / PEAK["fp64"],
@@ -543,6 +561,7 @@ def main():
                            "speedup": eps_res["fp64"]["kernel_s"] / eps_res[args.mixed]["kernel_s"],
                            "mixed_max_rel_err": eps_res[args.mixed].get("max_rel_frobenius_err")},
         "max_abs_omega_err_meV": max_err_mev,
+        "converged": {"mixed": res["mixed"]["converged"], "fp64": res.get("fp64", {}).get("converged")},
         "details": {"eps": eps_res, "bse": res},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -54,8 +54,9 @@ int xtpgpu_ctx_create(int device, unsigned flags, xtpgpu_ctx* out);
 int xtpgpu_ctx_destroy(xtpgpu_ctx ctx);
 int xtpgpu_ctx_synchronize(xtpgpu_ctx ctx);
 /* Live kernel timing: when enabled, HIP events are recorded on the context's stream around
- * the dominant launches (names: "rpa_eps_gemm", "hd_stage1" / "hd_stage2" for the BF16 forms
- * and "hd_stage1_fp64" / "hd_stage2_fp64" for FP64, "hd2", "hx", "hqp"), together with each
+ * the dominant launches (names: "rpa_eps_gemm", "hd_stage1" / "hd_stage2" for the split BF16x3
+ * form, "hd_stage1_bf16" / "hd_stage2_bf16" for plain BF16, "hd_stage1_fp64" / "hd_stage2_fp64"
+ * for FP64, "hd2", "hx", "hqp"), together with each
  * launch's algorithmic FLOPs and bytes. xtpgpu_ctx_timer synchronizes the stream and returns
  * the totals. */
 int xtpgpu_ctx_set_profiling(xtpgpu_ctx ctx, int enable);

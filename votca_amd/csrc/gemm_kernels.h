@@ -310,9 +310,11 @@ __device__ __forceinline__ void gemm_bf16_tile(const GemmDims& d, const Opnd<uns
                                                long s_bs, const Epi& epi, int tm, int tn, int b, int ks) {
   // BK = 64 (VEC only) gives every row a full 128-byte line per k-step: 8 lanes per row, 8 rows
   // per load instruction; the padded row (72 elements = 144 B) keeps MFMA fragment reads
-  // conflict-free.
+  // conflict-free. BK = 128 (plain BF16: one plane per operand) keeps the same LDS and register
+  // footprint as split BK = 64 and amortises each barrier over twice the MFMAs; its 272-byte
+  // padded rows are conflict-free as well.
   constexpr int BK = BKT;
-  static_assert(BK == 32 || (BK == 64 && VEC), "BK 64 needs vector loads");
+  static_assert(BK == 32 || ((BK == 64 || BK == 128) && VEC), "BK 64 / 128 need vector loads");
   constexpr int PK = BK + 8;  // padded LDS row (bf16 elements), rows stay 16-byte aligned
   static_assert(NT == 256 || NT == 512, "256 or 512 threads");
   constexpr int WM = BM / (NT / 128), WN = BN / 2;
@@ -604,20 +606,24 @@ inline hipError_t launch_gemm_bf16_tall(hipStream_t st, const GemmDims& d, Opnd<
   return hipGetLastError();
 }
 
+// K step of the wide-K vector path: 64 for split operands (two planes each), 128 for plain BF16.
+// d.kchunk of a launch with wide = true must be a multiple of it (make_dims with this bk).
+inline int bf16_wide_kstep(bool split) { return split ? 64 : 128; }
+
 template <int BM, int BN, bool AK, bool BK_, class Epi>
 inline hipError_t launch_gemm_bf16(hipStream_t st, const GemmDims& d, Opnd<unsigned short> A,
                                    Opnd<unsigned short> B, const float* s, long s_bs, bool split,
-                                   Epi epi, bool bk64 = false) {
+                                   Epi epi, bool wide = false) {
   dim3 grid(gemm_tiles(d), d.ksplit, d.batch);
   const bool vec = AK && BK_ && bf16_vec(A, B, d.K, s_bs);
   if constexpr (AK && BK_) {
-    if (vec && bk64) {
+    if (vec && wide) {
       if (split) {
         if (s) hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, AK, BK_, true, true, true, Epi, 64>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
         else hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, AK, BK_, true, false, true, Epi, 64>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
       } else {
-        if (s) hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, AK, BK_, false, true, true, Epi, 64>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
-        else hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, AK, BK_, false, false, true, Epi, 64>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+        if (s) hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, AK, BK_, false, true, true, Epi, 128>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+        else hipLaunchKernelGGL((gemm_bf16_kernel<BM, BN, AK, BK_, false, false, true, Epi, 128>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
       }
       return hipGetLastError();
     }

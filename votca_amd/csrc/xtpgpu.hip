@@ -215,7 +215,7 @@ void gemm16(hipStream_t st, int M, int N, int K, int batch, int ksplit, Opnd<uns
       return;
     }
     if ((choice == 1 || choice == 3 || choice == 4) && bf16_vec(A, B, K, s_bs)) {
-      GemmDims d = make_dims<128, 128>(M, N, std::max(K, 0), batch, ksplit, 64, lower);
+      GemmDims d = make_dims<128, 128>(M, N, std::max(K, 0), batch, ksplit, bf16_wide_kstep(split), lower);
       if (!lower) d.group = tile_group(d);
       XG_HIP((launch_gemm_bf16<128, 128, AK, BK_>(st, d, A, B, s, s_bs, split, epi, true)));
       return;
@@ -865,14 +865,14 @@ void bse_matmul_impl(xtpgpu_bseop op, int precision, const double* X, long ldx, 
         e1.Sh = Sh;
         e1.Sl = Sl;
         {
-          ProfScope prof(ctx, "hd_stage1", f1, b1);
+          ProfScope prof(ctx, split ? "hd_stage1" : "hd_stage1_bf16", f1, b1);
           gemm16<true, true>(st, (int)M1, (int)N1, (int)ctp, 1, 1,
                              Opnd<unsigned short>{op->Mcc_h.as<unsigned short>() + p0 * ct * ctp,
                                                   split ? op->Mcc_l.as<unsigned short>() + p0 * ct * ctp : nullptr,
                                                   ctp, 0},
                              Opnd<unsigned short>{Xh, Xl, ctp, 0}, nullptr, 0, split, false, e1);
         }
-        ProfScope prof(ctx, "hd_stage2", f2, b2);
+        ProfScope prof(ctx, split ? "hd_stage2" : "hd_stage2_bf16", f2, b2);
         const Opnd<unsigned short> s2a{Sh, Sl, k * np * vtp, np * vtp};
         const Opnd<unsigned short> s2b{op->Bvv_h.as<unsigned short>() + p0 * vtp,
                                        split ? op->Bvv_l.as<unsigned short>() + p0 * vtp : nullptr, P * vtp, 0};
@@ -883,7 +883,8 @@ void bse_matmul_impl(xtpgpu_bseop op, int precision, const double* X, long ldx, 
                                                                                (np * vtp) / 2048)));
         if (ks2 > 1 && bf16_vec(s2a, s2b, (int)(np * vtp), 0)) {
           // launched directly so the partial count is exactly dd.ksplit
-          const GemmDims dd = make_dims<128, 128>((int)ct, (int)vt, (int)(np * vtp), (int)k, ks2, 64, false);
+          const GemmDims dd = make_dims<128, 128>((int)ct, (int)vt, (int)(np * vtp), (int)k, ks2,
+                                                  bf16_wide_kstep(split), false);
           op->part.ensure((size_t)dd.ksplit * k * ct * vt * 4);
           float* pt = op->part.as<float>();
           XG_HIP((launch_gemm_bf16<128, 128, true, true>(st, dd, s2a, s2b, nullptr, 0, split, EpiPart32{pt, ct, vt, k},
