@@ -32,12 +32,16 @@ sys.path.insert(0, ROOT)
 
 HRT2EV = 27.21138602
 PEAK = {"fp64": 78.6e12, "bf16": 2.5e15, "hbm": 8.0e12}
-# HBM bytes of one hd_stage1 launch of the split-BF16 kernel (30-column H X, P-chunk of 263 levels,
-# 12 355 584 work-items), from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of
-# `bench.py --workload matvec --steps 1 --copy-calibration`, each scaled by what the same counter
-# reads for the 1 GiB copy16_kernel in that run (FETCH_SIZE 0.5 GiB -> x2.0, WRITE_SIZE 1.0 GiB -> x1.0).
-HD_STAGE1_TRAFFIC = {"read": 4.078 * 2.0 * 2**30, "write": 3.311 * 1.0 * 2**30,
-                     "launch": "30-column H X stage-1 launch (P-chunk 263)"}
+# HBM bytes of one hd_stage1 launch (30-column H X) of the split-BF16 and plain-BF16 kernels,
+# from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of
+# `bench.py --workload matvec --steps 1 [--mixed bf16] --copy-calibration`, each scaled by what the
+# same counter reads for the 1 GiB copy16_kernel in that run (FETCH_SIZE 0.5 GiB -> x2.0,
+# WRITE_SIZE 1.0 GiB -> x1.0). The P-chunk keeps the stage-1 intermediate under 3 GiB: 286 levels
+# for the split form (two output planes), 573 for plain BF16.
+HD_STAGE1_TRAFFIC = {"bf16x3": {"read": 4.078 * 2.0 * 2**30, "write": 3.311 * 1.0 * 2**30,
+                                "launch": "30-column H X stage-1 launch, split BF16 (P-chunk 286)"},
+                     "bf16": {"read": 2.254 * 2.0 * 2**30, "write": 2.908 * 1.0 * 2**30,
+                              "launch": "30-column H X stage-1 launch, plain BF16 (P-chunk 573)"}}
 
 
 def log(*a):
@@ -215,7 +219,7 @@ def run_matvec(args):
             op.matmul_device(X, k, Y, prec)
         ctx.synchronize()
         wall = (time.perf_counter() - t0) / args.steps
-        sfx = "_fp64" if prec == "fp64" else ""
+        sfx = {"fp64": "_fp64", "bf16": "_bf16"}.get(prec, "")
         tm = {name: ctx.timer(name + (sfx if name != "hx" else "")) for name in ("hd_stage1", "hd_stage2", "hx")}
         out[prec] = {"wall_ms": wall * 1e3,
                      **{f"{name}_tflops": t["flops"] / max(t["ms"] / 1e3, 1e-30) / 1e12 for name, t in tm.items()},
@@ -515,12 +519,15 @@ def main():
              "big": "gemm_bf16_kernel<256,256,..,EpiStage1,32>", "tall": "gemm_bf16_kernel<256,128,..,EpiStage1,64,512>"
              }.get(os.environ.get("XTPGPU_BF16_KERNEL", ""), "gemm_bf16_kernel<128,128,..,EpiStage1,64>")
     kname = kname.replace("<", "<" + ("split," if dom == "bf16x3" else "plain,"), 1)
-    traffic = (HD_STAGE1_TRAFFIC["read"] + HD_STAGE1_TRAFFIC["write"]
-               if kname.endswith(",64>") and dom == "bf16x3" else None)
+    if dom == "bf16" and kname.endswith(",64>") and "512" not in kname:
+        kname = kname[:-4] + ",128>"   # plain BF16 takes 128-wide K steps
+    traffic = (HD_STAGE1_TRAFFIC[dom]["read"] + HD_STAGE1_TRAFFIC[dom]["write"]
+               if dom in HD_STAGE1_TRAFFIC and "512" not in kname and "256,256" not in kname and "pipe" not in kname
+               else None)
     roofline = {"bound": "mfma", "kernel": f"hd_stage1 {dom} ({kname})",
                 "achieved": ach / 1e12,
                 "peak": PEAK["bf16"] / 1e12, "unit": "TFLOP/s", "frac": ach / PEAK["bf16"], "traffic": traffic,
-                "traffic_launch": HD_STAGE1_TRAFFIC["launch"] if traffic else None,
+                "traffic_launch": HD_STAGE1_TRAFFIC[dom]["launch"] if traffic else None,
                 "avg_launch_ms": kt * 1e3, "algorithmic_flops_per_launch": tm["flops"] / max(1, tm["launches"]),
                 "mfma_issued_tflops": issue * ach / 1e12, "mfma_issued_frac": issue * ach / PEAK["bf16"]}
     for other, t in stage1.items():
