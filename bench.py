@@ -267,7 +267,7 @@ def run_adjoint(args):
         ds.set_tolerance(args.tol)
         ds.set_correction("DPR")
         ds.set_size_update("safe")
-        ds.set_max_search_space(10 * args.roots)
+        ds.set_max_search_space(args.adj_space or 10 * args.roots)
         ds.set_iter_max(args.iter_max)
         t0 = time.perf_counter()
         ev, Xv, Yv = ds.solve_full(A, B, args.roots)
@@ -387,6 +387,7 @@ def main():
     ap.add_argument("--fd-levels", type=int, default=3)
     ap.add_argument("--fd-mmn", type=int, default=3)
     ap.add_argument("--fd-h", type=float, default=1e-5)
+    ap.add_argument("--adj-space", type=int, default=0, help="full-BSE max search space (default 10 * roots)")
     ap.add_argument("--cols", type=int, default=30)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="matvec workload: also run a 1 GiB copy kernel (counter calibration)")
