@@ -139,6 +139,51 @@ def test_bse_operator_fixtures_gpu(ctx, ints, name, c):  # test_bse_operator.cc:
     assert np.abs(H - ref).max() < 1e-12
 
 
+BSE_CASES = [  # test_bse.cc:40-330: (fixture, psi fixture, operator, use Hqp off-diagonals, roots, full BSE)
+    ("singlets_nooffdiag_tda", "singlets_psi_nooffdiag_tda", (1, 2, 1, 0), False, 3, False),
+    ("singlets_tda", "singlets_psi_tda", (1, 2, 1, 0), True, 3, False),
+    ("triplets_tda", None, (1, 0, 1, 0), True, 1, False),
+    ("singlets_btda", "singlets_psi_btda", (1, 2, 1, 0), True, 3, True),
+]
+
+
+@pytest.mark.parametrize("name,psi,c,offdiag,nroots,full", BSE_CASES, ids=[x[0] for x in BSE_CASES])
+def test_bse_energy_fixtures_gpu(ctx, ints, name, psi, c, offdiag, nroots, full):
+    """The reference's BSE test chain on the device: Fill3cMO with the 3-21G basis as its own
+    auxiliary basis, SetupDirectInteractionOperator at the RPA input energies diag(Hqp), then the
+    Davidson with the test's options ('lapack' tolerance 1e-9, DPR, 'safe', 50 iterations, search
+    space 10 * nmax): energies isApprox 1e-3, coefficient projections 1 within 1e-5."""
+    S, Saux, V, T = ints
+    C = mmio.read_matrix(os.path.join(G, "bse/MOs.mm"))
+    Hqp = mmio.read_matrix(os.path.join(G, "bse/Hqp.mm"))
+    t = px.TCMatrix_gwbse(ctx).Initialize(17, 0, 16, 0, 16)
+    t.Fill(T, C, X.pseudo_invsqrt_gwbse(V, Saux))
+    epsinv = px.setup_screening(t, np.diag(Hqp).copy(), 4, 0, 16)
+    Hq = Hqp if offdiag else np.diag(np.diag(Hqp))
+    opt = px.BSEOperator_Options(4, 0, 0, 0, 16)
+    ds = px.DavidsonSolver()
+    ds.set_tolerance("lapack")
+    ds.set_correction("DPR")
+    ds.set_size_update("safe")
+    ds.set_iter_max(50)
+    ds.set_max_search_space(10 * nroots)
+    A = px.BSE_OPERATOR(*c, epsinv, t, Hq).configure(opt)
+    if full:
+        B = px.BSE_OPERATOR(0, c[1], 0, 1, epsinv, t, Hq).configure(opt)
+        ev, Xv, Yv = ds.solve_full(A, B, nroots)
+    else:
+        ev, Xv = ds.solve(A, nroots)
+    assert ds.converged(), ds.info.max_residual
+    assert mmio.isapprox(ev, mmio.read_matrix(os.path.join(G, f"bse/{name}.mm")).ravel(), 1e-3)
+    unit = lambda a: a / np.linalg.norm(a, axis=0)  # noqa: E731
+    if psi:
+        R = unit(mmio.read_matrix(os.path.join(G, f"bse/{psi}.mm")))
+        assert np.abs(np.linalg.norm(R.T @ unit(Xv), axis=0) - 1).max() < 1e-5
+    if full:
+        RA = unit(mmio.read_matrix(os.path.join(G, "bse/singlets_psi_AR_btda.mm")))
+        assert np.abs(np.linalg.norm(RA.T @ unit(Yv), axis=0) - 1).max() < 1e-5
+
+
 def random_bse_case(seed, vt=7, ct=33, P=70, rpamin=0, vmin_off=0, offdiag=False, scale=0.3):
     rng = np.random.default_rng(seed)
     ntot = vmin_off + vt + ct

@@ -232,3 +232,53 @@ def test_sigma_cda_fixture(ints):  # test_sigma_cda.cc:41-113 (Legendre order 10
     cda = X.SigmaCDAOracle(M, CDA_E, 4, 0, 16, 0, 1e-3, 1e-3, q.points(), q.weights(), q.use_symmetry())
     c = np.array([cda.diag(CDA_E[l], l) for l in range(17)])
     assert mmio.isapprox(c, np.diag(mmio.read_matrix(os.path.join(G, "sigma_cda/c_ref.mm"))), 1e-5)
+
+
+BSE_CASES = [  # test_bse.cc:40-330: (fixture, psi fixture, operator, use Hqp off-diagonals, roots, full BSE)
+    ("singlets_nooffdiag_tda", "singlets_psi_nooffdiag_tda", (1, 2, 1, 0), False, 3, False),
+    ("singlets_tda", "singlets_psi_tda", (1, 2, 1, 0), True, 3, False),
+    ("triplets_tda", None, (1, 0, 1, 0), True, 1, False),  # the reference skips the triplet psi check
+    ("singlets_btda", "singlets_psi_btda", (1, 2, 1, 0), True, 3, True),
+]
+
+
+@pytest.fixture(scope="module")
+def bse_chain(ints):
+    """test_bse.cc:40-69: Mmn from the 3-21G basis used as its own auxiliary basis, screening
+    from eps_r(0) at the RPA input energies diag(Hqp) (BSE::SetupDirectInteractionOperator)."""
+    C = mmio.read_matrix(os.path.join(G, "bse/MOs.mm"))
+    Hqp = mmio.read_matrix(os.path.join(G, "bse/Hqp.mm"))
+    S, Saux, V, T = ints
+    M = X.fill_mmn(T, C, 0, 16, 0, 16, Saux, V)
+    eps = X.rpa_epsilon(M, np.diag(Hqp).copy(), 4, 0, 16, 0.0, "real")
+    lam, U = np.linalg.eigh(eps)
+    Mr = np.einsum("mnp,pq->mnq", M, U)
+    return Mr, np.where(lam > 1e-8, 1.0 / lam, 0.0), Hqp
+
+
+def unit_columns(A):
+    return A / np.linalg.norm(A, axis=0)
+
+
+@pytest.mark.parametrize("name,psi,c,offdiag,nroots,full", BSE_CASES, ids=[x[0] for x in BSE_CASES])
+def test_bse_energy_fixtures(bse_chain, name, psi, c, offdiag, nroots, full):
+    """Excitation energies (isApprox 1e-3) and coefficient projections (1 within 1e-5) of the
+    reference's BSE test from the oracle's dense BSE matrices."""
+    Mr, epsinv, Hqp = bse_chain
+    Hq = Hqp if offdiag else np.diag(np.diag(Hqp))
+    A = X.bse_dense(*c, epsinv, Mr, Hq, 4, 0, 0, 16)
+    if full:
+        B = X.bse_dense(0, c[1], 0, 1, epsinv, Mr, Hq, 4, 0, 0, 16)
+        w, v = np.linalg.eig(np.block([[A, B], [-B, -A]]))
+        idx = [i for i in np.argsort(w.real) if w.real[i] > 0][:nroots]
+        ev, Xv, Yv = w.real[idx], v[:A.shape[0], idx].real, v[A.shape[0]:, idx].real
+    else:
+        w, v = np.linalg.eigh(A)
+        ev, Xv, Yv = w[:nroots], v[:, :nroots], None
+    assert mmio.isapprox(ev, mmio.read_matrix(os.path.join(G, f"bse/{name}.mm")).ravel(), 1e-3)
+    if psi:
+        R = unit_columns(mmio.read_matrix(os.path.join(G, f"bse/{psi}.mm")))
+        assert np.abs(np.linalg.norm(R.T @ unit_columns(Xv), axis=0) - 1).max() < 1e-5
+    if full:
+        RA = unit_columns(mmio.read_matrix(os.path.join(G, "bse/singlets_psi_AR_btda.mm")))
+        assert np.abs(np.linalg.norm(RA.T @ unit_columns(Yv), axis=0) - 1).max() < 1e-5
