@@ -367,6 +367,111 @@ class DavidsonOracle:
         return self.eigenvalues, self.eigenvectors
 
 
+    def solve_ham(self, matmul, diag_a, diag_b, neigen):
+        """HAM matrix type as called by BSE::Solve_nonhermitian_Davidson (bse.cc:320-358): guess
+        from BuildFullBSEXRankedInitialGuess (bse_initialization.h:47-93), diagonal [Ad; -Ad]
+        (bseoperator_btda.h:85-91), projections T = V^T H V and B = V^T H H V (davidsonsolver.h:
+        239-280), harmonic Ritz pairs (davidsonsolver.cc:241-332) and the QR restart
+        (davidsonsolver.cc:499-518). `matmul` applies H = [[A, B], [-B, -A]] to 2N x k blocks."""
+        import scipy.linalg as sla
+        diag_a = np.asarray(diag_a, dtype=np.float64)
+        diag_b = np.asarray(diag_b, dtype=np.float64)
+        nh = diag_a.size
+        n = 2 * nh
+        diag = np.concatenate([diag_a, -diag_a])
+        if self.max_search_space < neigen:
+            self.max_search_space = neigen * 5
+        if self.max_search_space > n:
+            self.max_search_space = n
+        nguess = min(nh, max(4 * neigen, 8))
+        omega = np.sqrt(np.maximum(0.0, (diag_a - diag_b) * (diag_a + diag_b)))
+        rank = sorted(range(nh), key=lambda i: (omega[i], diag_a[i]))
+        V = np.zeros((n, nguess))
+        for j in range(nguess):
+            V[rank[j], j] = 1.0
+        V = self.gramschmidt(V, 0)
+        restart_size = min(nguess, self.max_search_space)
+        size_update = self.size_update(neigen)
+        self.info_converged = False
+        self.history = []
+        AV = AAV = None
+        for it in range(self.iter_max):
+            self.num_iterations = it
+            old = 0 if AV is None else AV.shape[1]
+            AVn = matmul(V[:, old:])
+            AAVn = matmul(AVn)
+            AV = AVn if AV is None else np.hstack([AV, AVn])
+            AAV = AAVn if AAV is None else np.hstack([AAV, AAVn])
+            T = V.T @ AV
+            B = V.T @ AAV
+            w, vr = sla.eig(T, B)
+            keep, taken = [], set()
+            for i in range(len(w)):  # first member of each complex pair kept (real part)
+                if w[i].imag != 0:
+                    if i in taken:
+                        continue
+                    for k in range(i + 1, len(w)):
+                        if k not in taken and abs(w[k].real - w[i].real) < 1e-9 and abs(w[k].imag + w[i].imag) < 1e-9:
+                            taken.add(k)
+                            break
+                keep.append(i)
+            ev = w.real[keep]
+            U = vr.real[:, keep]
+            U = U / np.linalg.norm(U, axis=0)
+            need = min(T.shape[1], max(restart_size, size_update))
+            order = np.argsort(ev, kind="stable")[::-1][:need]
+            U = U[:, order]
+            lam = np.einsum("ij,ik,kj->j", U, T, U)
+            q = V @ U
+            res = AV @ U - q * lam[None, :]
+            rn = np.linalg.norm(res, axis=0)
+            self.history.append(float(rn[:neigen].max()))
+            conv = rn[:size_update] < self.tol
+            if np.all(conv[:neigen]):
+                self.info_converged = True
+                break
+            if it == self.iter_max - 1:
+                lam = lam.copy()
+                q = q.copy()
+                for i in range(neigen):
+                    if not conv[i]:
+                        lam[i] = 0.0
+                        q[:, i] = 0.0
+                break
+            newv = []
+            for j in range(size_update):
+                if conv[j]:
+                    continue
+                if self.correction == "DPR":
+                    wv = -res[:, j] / (diag - lam[j])
+                else:
+                    d = -res[:, j] / (diag - lam[j])
+                    xq = q[:, j]
+                    wv = d + ((-xq @ d) / (-xq @ (-xq / (diag - lam[j])))) * xq
+                wv = np.where(np.isfinite(wv), wv, 0.0)
+                newv.append(wv / np.linalg.norm(wv))
+            nup = len(newv)
+            V = np.hstack([V, np.array(newv).T])
+            V = self.gramschmidt(V, V.shape[1] - nup)
+            if V.shape[1] > self.max_search_space:
+                Q, _ = np.linalg.qr(U[:, :restart_size])
+                V = np.hstack([V[:, :V.shape[1] - nup] @ Q, V[:, -nup:]])
+                AV = AV @ Q
+                AAV = AAV @ Q
+        self.eigenvalues = lam[:neigen]
+        vecs = q[:, :neigen]
+        nrm = np.linalg.norm(vecs, axis=0)
+        nrm[nrm == 0] = 1.0
+        vecs = vecs / nrm
+        nx = np.sum(vecs[:nh] ** 2, axis=0)
+        ny = np.sum(vecs[nh:] ** 2, axis=0)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            s = np.sqrt(1.0 / (nx - ny))
+        self.X = vecs[:nh] * s
+        self.Y = vecs[nh:] * s
+        return self.eigenvalues, self.X, self.Y
+
+
 # ---------------------------------------------------------------- synthetic inputs
 _M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
 
@@ -621,3 +726,38 @@ class SigmaCDAOracle:
 
     def diag(self, omega, level):
         return self.residues(omega, level) + self.sigma_gq(omega, level)
+
+
+def screening_at(M, rpa_energies, homo, rpamin, rpamax, omega):
+    """BSE::SetupDirectInteractionOperator(energies, omega) (bse.cc:188-204): eps_r(omega) of the
+    RPA energies, M rotated to its eigenvectors, eps^-1 = 1/lambda (lambda > 1e-8, else 0)."""
+    eps = rpa_epsilon(M, rpa_energies, homo, rpamin, rpamax, omega, "real")
+    lam, U = np.linalg.eigh(eps)
+    return np.einsum("mnp,pq->mnq", M, U), np.where(lam > 1e-8, 1.0 / lam, 0.0)
+
+
+def perturbative_dynamical_screening(M, rpa_energies, homo, rpamin, rpamax, Hqp, vmin, cmax, energies, X, Y=None,
+                                     max_iter=10, tol=1e-5):
+    """BSE::Perturbative_DynamicalScreening (bse.cc:610-700): per excitation, iterate
+    Omega_dyn = Omega + <Hd>(W(0)) - <Hd>(W(Omega_dyn)) from Omega_dyn = Omega until the change is
+    below `tol` (at most `max_iter` passes). <Hd> is ExpectationValue_Operator(_State)
+    (bse.cc:500-533): X^T Hd X, plus Y^T Hd Y + 2 Y^T Hd2 X for the full BSE."""
+    def direct(Mr, epsinv, Xs, Ys):
+        Hd = bse_dense(0, 0, 1, 0, epsinv, Mr, Hqp, homo, rpamin, vmin, cmax)
+        d = np.sum(Xs * (Hd @ Xs), axis=0)
+        if Ys is not None:
+            Hd2 = bse_dense(0, 0, 0, 1, epsinv, Mr, Hqp, homo, rpamin, vmin, cmax)
+            d = d + np.sum(Ys * (Hd @ Ys), axis=0) + 2.0 * np.sum(Ys * (Hd2 @ Xs), axis=0)
+        return d
+
+    static = direct(*screening_at(M, rpa_energies, homo, rpamin, rpamax, 0.0), X, Y)
+    out = np.array(energies, dtype=float).copy()
+    for i in range(len(out)):
+        for _ in range(max_iter):
+            old = out[i]
+            Mr, epsinv = screening_at(M, rpa_energies, homo, rpamin, rpamax, old)
+            dyn = direct(Mr, epsinv, X[:, i:i + 1], None if Y is None else Y[:, i:i + 1])[0]
+            out[i] = energies[i] + static[i] - dyn
+            if abs(out[i] - old) < tol:
+                break
+    return out

@@ -184,6 +184,49 @@ def test_bse_energy_fixtures_gpu(ctx, ints, name, psi, c, offdiag, nroots, full)
         assert np.abs(np.linalg.norm(RA.T @ unit(Yv), axis=0) - 1).max() < 1e-5
 
 
+
+DYN_CASES = [  # test_bse.cc:177-191, 268-282, 326-340: (fixture, operator, roots, full BSE, isApprox tolerance)
+    ("singlets_dynamic_TDA", (1, 2, 1, 0), 3, False, 0.005),
+    ("triplets_dynamic_TDA", (1, 0, 1, 0), 1, False, 0.001),
+    ("singlets_dynamic_full", (1, 2, 1, 0), 3, True, 0.05),
+]
+
+
+@pytest.mark.parametrize("name,c,nroots,full,tol", DYN_CASES, ids=[x[0] for x in DYN_CASES])
+def test_dynamical_screening_fixtures_gpu(ctx, ints, name, c, nroots, full, tol):
+    """BSE::Perturbative_DynamicalScreening on the device (screening W(Omega) and the Hd/Hd2
+    operators), fed by the device Davidson's eigenpairs of the reference's test: the fixture at the
+    test's isApprox tolerance, and the oracle on the same eigenvectors to 1e-9."""
+    S, Saux, V, T = ints
+    C = mmio.read_matrix(os.path.join(G, "bse/MOs.mm"))
+    Hqp = mmio.read_matrix(os.path.join(G, "bse/Hqp.mm"))
+    e = np.diag(Hqp).copy()
+    t = px.TCMatrix_gwbse(ctx).Initialize(17, 0, 16, 0, 16)
+    t.Fill(T, C, X.pseudo_invsqrt_gwbse(V, Saux))
+    epsinv = px.setup_screening(t, e, 4, 0, 16)
+    opt = px.BSEOperator_Options(4, 0, 0, 0, 16)
+    ds = px.DavidsonSolver()
+    ds.set_tolerance("lapack")
+    ds.set_correction("DPR")
+    ds.set_size_update("safe")
+    ds.set_iter_max(50)
+    ds.set_max_search_space(10 * nroots)
+    A = px.BSE_OPERATOR(*c, epsinv, t, Hqp).configure(opt)
+    if full:
+        B = px.BSE_OPERATOR(0, c[1], 0, 1, epsinv, t, Hqp).configure(opt)
+        ev, Xv, Yv = ds.solve_full(A, B, nroots)
+        Xv = Xv / np.linalg.norm(Xv, axis=0)
+        Yv = Yv / np.linalg.norm(Yv, axis=0)
+    else:
+        ev, Xv = ds.solve(A, nroots)
+        Yv = None
+    assert ds.converged(), ds.info.max_residual
+    dyn = px.Perturbative_DynamicalScreening(t, e, Hqp, opt, 16, ev, Xv, Yv)
+    assert mmio.isapprox(dyn, mmio.read_matrix(os.path.join(G, f"bse/{name}.mm")).ravel(), tol)
+    M = X.fill_mmn(T, C, 0, 16, 0, 16, Saux, V)
+    ref = X.perturbative_dynamical_screening(M, e, 4, 0, 16, Hqp, 0, 16, ev, Xv, Yv)
+    assert np.abs(dyn - ref).max() < 1e-9
+
 def random_bse_case(seed, vt=7, ct=33, P=70, rpamin=0, vmin_off=0, offdiag=False, scale=0.3):
     rng = np.random.default_rng(seed)
     ntot = vmin_off + vt + ct

@@ -34,13 +34,25 @@ int main() {
       for (int i = 0; i < need; ++i) std::printf("%.17g ", w[i]);
       std::printf("\n");
       for (size_t i = 0; i < Z.size(); ++i) std::printf("%.17g ", Z[i]);
-    } else {
+    } else if (kind == 2) {
       std::vector<double> wr(n), wi(n);
       int ok = xg::nonsym_eigvals(n, A.data(), n, wr.data(), wi.data());
       std::printf("%d\n", ok);
       for (int i = 0; i < n; ++i) std::printf("%.17g ", wr[i]);
       std::printf("\n");
       for (int i = 0; i < n; ++i) std::printf("%.17g ", wi[i]);
+    } else {
+      // eigenvalue `need` given as (wr, wi) after the matrix: real part of its eigenvector
+      double wr, wi;
+      std::scanf("%lf %lf", &wr, &wi);
+      std::vector<double> x(n);
+      if (wi == 0.0)
+        xg::real_eigvec(n, A.data(), n, wr, x.data());
+      else
+        xg::complex_eigvec_real(n, A.data(), n, wr, wi, x.data());
+      std::printf("1\n");
+      for (int i = 0; i < n; ++i) std::printf("%.17g ", x[i]);
+      std::printf("\n");
     }
     std::printf("\n");
   }
@@ -61,9 +73,9 @@ def driver(tmp_path_factory):
     return str(exe)
 
 
-def run(driver, kind, A, need=0):
+def run(driver, kind, A, need=0, extra=""):
     n = A.shape[0]
-    inp = f"{kind} {n} {need}\n" + " ".join(repr(float(x)) for x in A.flatten(order="F")) + "\n"
+    inp = f"{kind} {n} {need}\n" + " ".join(repr(float(x)) for x in A.flatten(order="F")) + "\n" + extra
     out = subprocess.run([driver], input=inp, capture_output=True, text=True, check=True).stdout.split("\n")
     ok = int(out[0])
     a = np.array(out[1].split(), dtype=float)
@@ -114,3 +126,25 @@ def test_nonsymmetric_eigenvalues_match_lapack(driver):
         got = np.sort_complex(wr + 1j * wi)
         ref = np.sort_complex(np.linalg.eigvals(A))
         assert np.abs(got - ref).max() < 1e-10 * max(1.0, np.abs(ref).max())
+
+
+def test_eigenvector_real_parts_match_lapack(driver):
+    """Right eigenvectors for real eigenvalues and the real parts of those for complex ones (unit
+    norm, largest-modulus component real: xGEEV's convention), as the harmonic Ritz step keeps
+    them, against numpy's LAPACK eig up to sign. Includes a near-degenerate real triple."""
+    rng = np.random.default_rng(9)
+    mats = [rng.normal(size=(n, n)) for n in (2, 7, 30)]
+    Q = np.linalg.qr(rng.normal(size=(6, 6)))[0]
+    D = np.diag([1.0, 1.0 + 1e-6, 1.0 + 2e-6, 3.0, -2.0, 5.0])
+    mats.append(Q @ D @ np.linalg.inv(Q + 0.1 * rng.normal(size=(6, 6))) @ (Q + 0.1 * rng.normal(size=(6, 6))) @ Q.T)
+    ncomplex = 0
+    for A in mats:
+        w, V = np.linalg.eig(A)
+        for i in range(len(w)):
+            if w[i].imag < 0:
+                continue
+            ncomplex += w[i].imag != 0
+            _, x, _ = run(driver, 3, A, 0, f"{float(w[i].real)!r} {float(w[i].imag)!r}\n")
+            ref = V[:, i].real / np.linalg.norm(V[:, i].real)
+            assert abs(abs(x @ ref) - 1.0) < 1e-8, (A.shape, w[i])
+    assert ncomplex > 0

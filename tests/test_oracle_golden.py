@@ -282,3 +282,62 @@ def test_bse_energy_fixtures(bse_chain, name, psi, c, offdiag, nroots, full):
     if full:
         RA = unit_columns(mmio.read_matrix(os.path.join(G, "bse/singlets_psi_AR_btda.mm")))
         assert np.abs(np.linalg.norm(RA.T @ unit_columns(Yv), axis=0) - 1).max() < 1e-5
+
+
+def test_ham_davidson_oracle_fixture(bse_chain):
+    """The HAM-type Davidson (harmonic Ritz, X-ranked guess) with the reference test's options
+    ('lapack' 1e-9, DPR, 'safe', 50 iterations, search space 10 * nmax) converges on the full BSE
+    of the test molecule to the dense eigenpairs and to the singlets_btda fixtures."""
+    Mr, epsinv, Hqp = bse_chain
+    A = X.bse_dense(1, 2, 1, 0, epsinv, Mr, Hqp, 4, 0, 0, 16)
+    B = X.bse_dense(0, 2, 0, 1, epsinv, Mr, Hqp, 4, 0, 0, 16)
+    H = np.block([[A, B], [-B, -A]])
+    d = X.DavidsonOracle(tol="lapack", correction="DPR", update="safe", iter_max=50, max_search_space=30)
+    ev, Xv, Yv = d.solve_ham(lambda v: H @ v, np.diag(A), np.diag(B), 3)
+    assert d.info_converged
+    w = np.linalg.eigvals(H).real
+    assert np.abs(ev - np.sort(w[w > 0])[:3]).max() < 1e-9
+    assert mmio.isapprox(ev, mmio.read_matrix(os.path.join(G, "bse/singlets_btda.mm")).ravel(), 1e-3)
+    # |X|^2 - |Y|^2 = 1 and the eigen-equation of the doubled space
+    assert np.allclose(np.sum(Xv ** 2, 0) - np.sum(Yv ** 2, 0), 1.0, atol=1e-12)
+    Z = np.vstack([Xv, Yv])
+    assert np.abs(H @ Z - Z * ev).max() < 1e-8
+    R = unit_columns(mmio.read_matrix(os.path.join(G, "bse/singlets_psi_btda.mm")))
+    assert np.abs(np.linalg.norm(R.T @ unit_columns(Xv), axis=0) - 1).max() < 1e-5
+    RA = unit_columns(mmio.read_matrix(os.path.join(G, "bse/singlets_psi_AR_btda.mm")))
+    assert np.abs(np.linalg.norm(RA.T @ unit_columns(Yv), axis=0) - 1).max() < 1e-5
+
+
+DYN_CASES = [  # test_bse.cc:177-191, 268-282, 326-340: (fixture, operator, roots, full BSE, isApprox tolerance)
+    ("singlets_dynamic_TDA", (1, 2, 1, 0), 3, False, 0.005),
+    ("triplets_dynamic_TDA", (1, 0, 1, 0), 1, False, 0.001),
+    ("singlets_dynamic_full", (1, 2, 1, 0), 3, True, 0.05),
+]
+
+
+@pytest.mark.parametrize("name,c,nroots,full,tol", DYN_CASES, ids=[x[0] for x in DYN_CASES])
+def test_dynamical_screening_fixtures(ints, bse_chain, name, c, nroots, full, tol):
+    """Perturbative dynamical screening (bse.cc:610-700) of the reference's test: static BSE
+    eigenpairs, then the self-consistent Omega + <Hd>(W(0)) - <Hd>(W(Omega)) (max_dyn_iter 10,
+    dyn_tolerance 1e-5). For the full BSE the test normalizes X and Y to unit columns first
+    (test_bse.cc:220,251)."""
+    S, Saux, V, T = ints
+    C = mmio.read_matrix(os.path.join(G, "bse/MOs.mm"))
+    Mr, epsinv, Hqp = bse_chain
+    M = X.fill_mmn(T, C, 0, 16, 0, 16, Saux, V)
+    e = np.diag(Hqp).copy()
+    A = X.bse_dense(*c, epsinv, Mr, Hqp, 4, 0, 0, 16)
+    if full:
+        B = X.bse_dense(0, c[1], 0, 1, epsinv, Mr, Hqp, 4, 0, 0, 16)
+        w, v = np.linalg.eig(np.block([[A, B], [-B, -A]]))
+        idx = [i for i in np.argsort(w.real) if w.real[i] > 0][:nroots]
+        ev, Xv, Yv = w.real[idx], unit_columns(v[:A.shape[0], idx].real), unit_columns(v[A.shape[0]:, idx].real)
+    else:
+        w, v = np.linalg.eigh(A)
+        ev, Xv, Yv = w[:nroots], v[:, :nroots], None
+    dyn = X.perturbative_dynamical_screening(M, e, 4, 0, 16, Hqp, 0, 16, ev, Xv, Yv)
+    assert mmio.isapprox(dyn, mmio.read_matrix(os.path.join(G, f"bse/{name}.mm")).ravel(), tol)
+    # zero iterations leave the static energies; at Omega = 0 the dynamic term is the static one
+    assert np.array_equal(X.perturbative_dynamical_screening(M, e, 4, 0, 16, Hqp, 0, 16, ev, Xv, Yv, max_iter=0), ev)
+    zero = X.perturbative_dynamical_screening(M, e, 4, 0, 16, Hqp, 0, 16, np.zeros_like(ev), Xv, Yv)
+    assert np.abs(zero).max() < 1e-12

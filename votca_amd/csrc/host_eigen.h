@@ -7,6 +7,7 @@
 #pragma once
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <numeric>
 #include <stdexcept>
 #include <vector>
@@ -505,6 +506,63 @@ inline void real_eigvec(int n, const double* A, int lda, double lam, double* x) 
     nr = std::sqrt(nr);
     for (int i = 0; i < n; ++i) x[i] /= nr;
   }
+}
+
+// Real part of the right eigenvector of A for a complex eigenvalue wr + i wi (wi != 0), by
+// complex inverse iteration on (A - mu I), mu perturbed by a relative 1e-10. The complex vector is
+// taken with unit Euclidean norm and its largest-modulus component real (LAPACK xGEEV's
+// convention); x receives its real part, normalized to unit length.
+inline void complex_eigvec_real(int n, const double* A, int lda, double wr, double wi, double* x) {
+  using cd = std::complex<double>;
+  std::vector<cd> M((size_t)n * n);
+  const double scale = std::max(1.0, std::hypot(wr, wi));
+  const cd shift(wr + 1e-10 * scale, wi);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) M[(size_t)j * n + i] = cd(A[(size_t)j * lda + i]) - (i == j ? shift : cd(0.0));
+  std::vector<int> piv(n);
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    for (int i = k + 1; i < n; ++i)
+      if (std::abs(M[(size_t)k * n + i]) > std::abs(M[(size_t)k * n + p])) p = i;
+    piv[k] = p;
+    if (p != k)
+      for (int j = 0; j < n; ++j) std::swap(M[(size_t)j * n + k], M[(size_t)j * n + p]);
+    cd d = M[(size_t)k * n + k];
+    if (d == cd(0.0)) d = M[(size_t)k * n + k] = cd(1e-300);
+    for (int i = k + 1; i < n; ++i) M[(size_t)k * n + i] /= d;
+    for (int j = k + 1; j < n; ++j) {
+      const cd f = M[(size_t)j * n + k];
+      if (f != cd(0.0))
+        for (int i = k + 1; i < n; ++i) M[(size_t)j * n + i] -= M[(size_t)k * n + i] * f;
+    }
+  }
+  std::vector<cd> z(n);
+  for (int i = 0; i < n; ++i) z[i] = cd(1.0 / std::sqrt((double)n) * (1.0 + 0.01 * (double)(i % 7)), 0.0);
+  for (int it = 0; it < 3; ++it) {
+    for (int k = 0; k < n; ++k) std::swap(z[k], z[piv[k]]);
+    for (int j = 0; j < n; ++j)
+      for (int i = j + 1; i < n; ++i) z[i] -= M[(size_t)j * n + i] * z[j];
+    for (int j = n - 1; j >= 0; --j) {
+      z[j] /= M[(size_t)j * n + j];
+      for (int i = 0; i < j; ++i) z[i] -= M[(size_t)j * n + i] * z[j];
+    }
+    double nr = 0.0;
+    for (int i = 0; i < n; ++i) nr += std::norm(z[i]);
+    nr = std::sqrt(nr);
+    for (int i = 0; i < n; ++i) z[i] /= nr;
+  }
+  int kmax = 0;
+  for (int i = 1; i < n; ++i)
+    if (std::abs(z[i]) > std::abs(z[kmax])) kmax = i;
+  const cd phase = std::conj(z[kmax]) / std::abs(z[kmax]);
+  double nr = 0.0;
+  for (int i = 0; i < n; ++i) {
+    x[i] = (z[i] * phase).real();
+    nr += x[i] * x[i];
+  }
+  nr = std::sqrt(nr);
+  if (nr > 0.0)
+    for (int i = 0; i < n; ++i) x[i] /= nr;
 }
 
 }  // namespace xg

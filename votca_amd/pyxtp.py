@@ -565,6 +565,47 @@ def setup_screening(Mmn: TCMatrix_gwbse, energies, homo, rpamin, rpamax, energy=
     return out
 
 
+def _expectation(Hd: BSE_OPERATOR, X, Y=None, Hd2: BSE_OPERATOR | None = None) -> np.ndarray:
+    """BSE::ExpectationValue_Operator(_State) (bse.cc:500-547) summed as the dynamical screening
+    uses it: X^T Hd X, plus Y^T Hd Y + 2 Y^T Hd2 X for the full BSE (column by column)."""
+    d = np.sum(X * Hd.matmul(X), axis=0)
+    if Y is not None:
+        d = d + np.sum(Y * Hd.matmul(Y), axis=0) + 2.0 * np.sum(Y * Hd2.matmul(X), axis=0)
+    return d
+
+
+def Perturbative_DynamicalScreening(Mmn: TCMatrix_gwbse, rpa_energies, Hqp, opt: BSEOperator_Options, rpamax,
+                                    energies, X, Y=None, max_dyn_iter=10, dyn_tolerance=1e-5) -> np.ndarray:
+    """BSE::Perturbative_DynamicalScreening (bse.cc:610-700) with the device screening and Hd/Hd2
+    operators: static <Hd> from W(0), then per excitation
+    Omega_dyn = Omega + <Hd>(W(0)) - <Hd>(W(Omega_dyn)), restarted from the static energy and
+    iterated until the change is below `dyn_tolerance` (at most `max_dyn_iter` passes). Y given
+    selects the full BSE (cross term with Hd2). Like the reference, every screening set-up rotates
+    `Mmn` in place (the excitation energies do not depend on the rotation)."""
+    X = _f64(X)
+    Y = None if Y is None else _f64(Y)
+    e = np.ascontiguousarray(rpa_energies, dtype=np.float64)
+
+    def ops(omega):
+        eps = setup_screening(Mmn, e, opt.homo, opt.rpamin, rpamax, omega)
+        hd = HdOperator(eps, Mmn, Hqp).configure(opt)
+        hd2 = Hd2Operator(eps, Mmn, Hqp).configure(opt) if Y is not None else None
+        return hd, hd2
+
+    hd, hd2 = ops(0.0)
+    static = _expectation(hd, X, Y, hd2)
+    out = np.array(energies, dtype=np.float64).copy()
+    for i in range(out.size):
+        for _ in range(int(max_dyn_iter)):
+            old = out[i]
+            hd, hd2 = ops(old)
+            dyn = _expectation(hd, X[:, i:i + 1], None if Y is None else Y[:, i:i + 1], hd2)[0]
+            out[i] = energies[i] + static[i] - dyn
+            if abs(out[i] - old) < dyn_tolerance:
+                break
+    return out
+
+
 class Sigma_PPM:
     """Sigma_PPM (sigma_ppm.h) + Sigma_base exchange (sigma_base.h:33-106) on the device."""
 
