@@ -156,6 +156,14 @@ int xtpgpu_bse_setup_screening(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* ene
  * = -sum_{n<occ,P} M^P_{l1 n} M^P_{l2 n}. */
 int xtpgpu_sigma_x(xtpgpu_ctx ctx, xtpgpu_mmn mmn, int64_t homo, int64_t rpamin, int64_t qpmin,
                    int64_t qpmax, double* out);
+/* RPA::Diagonalize_H2p (rpa.cc:204-258, Calculate_H2p_AmB/ApB rpa.cc:260-331, Diagonalize_H2p_C
+ * rpa.cc:333-345): C = (A-B)^1/2 (A+B) (A-B)^1/2 with (A-B)_{I(v,c)} = e_c - e_v and
+ * A+B = diag(A-B) + 4 V V^T, V_{I(v,c),P} = M^P_{vc}; energies are the RPA input energies from
+ * rpamin. omega (N = n_occ * n_unocc) ascending; xpy (N x N, column-major, may be NULL) the X+Y
+ * vectors; erpa (may be NULL) the RPA correlation energy. XTPGPU_ERR_NUMERIC if an eigenvalue
+ * of C is not positive. */
+int xtpgpu_rpa_h2p(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* energies, int64_t homo, int64_t rpamin,
+                   int64_t rpamax, double* omega, double* xpy, double* erpa);
 /* PPM::PPM_construct_parameters (ppm.cc:31-59) + Sigma_PPM::PrepareScreening (sigma_ppm.cc:32-35):
  * eps_r(0), eps_i(0.5) on the device, phi = eigenvectors of eps_r(0), weight = 1 - 1/lambda,
  * freq from diag((phi^T eps_i phi)^-1); then Mmn <- Mmn * phi in place. */

@@ -275,6 +275,75 @@ def coulomb3c(auxshells, shells) -> np.ndarray:
     return T
 
 
+# nuclear charges of the elements the fixtures use (all-electron basis sets, no ECP)
+NUCLEAR_CHARGE = {"H": 1.0, "He": 2.0, "C": 6.0, "N": 7.0, "O": 8.0, "F": 9.0}
+
+
+def _overlap_1d(i: int, j: int, qx: float, a: float, b: float) -> float:
+    return _E(i, j, 0, qx, a, b) * math.sqrt(math.pi / (a + b)) if j >= 0 else 0.0
+
+
+def _kinetic_1d(i: int, j: int, qx: float, a: float, b: float) -> float:
+    """<x^i e^{-a x^2}| -1/2 d^2/dx^2 |x^j e^{-b x^2}> from
+    d^2/dx^2 x^j e^{-b x^2} = [j(j-1) x^{j-2} - 2b(2j+1) x^j + 4b^2 x^{j+2}] e^{-b x^2}."""
+    t = -2.0 * b * (2 * j + 1) * _overlap_1d(i, j, qx, a, b) + 4.0 * b * b * _overlap_1d(i, j + 2, qx, a, b)
+    if j >= 2:
+        t += j * (j - 1) * _overlap_1d(i, j - 2, qx, a, b)
+    return -0.5 * t
+
+
+def kinetic_matrix(shells) -> np.ndarray:
+    """T_{mu nu} = <mu| -1/2 nabla^2 |nu> (the reference's AOKinetic, aokinetic.cc)."""
+    off, n = _offsets(shells)
+    K = np.zeros((n, n))
+    for i, si in enumerate(shells):
+        for j, sj in enumerate(shells):
+            blk = np.zeros((si.nfunc, sj.nfunc))
+            Q = si.center - sj.center
+            for a, ca in zip(si.exps, si.coefs):
+                for b, cb in zip(sj.exps, sj.coefs):
+                    for x, c1 in enumerate(_CART[si.l]):
+                        for y, c2 in enumerate(_CART[sj.l]):
+                            S1 = [_overlap_1d(c1[d], c2[d], Q[d], a, b) for d in range(3)]
+                            T1 = [_kinetic_1d(c1[d], c2[d], Q[d], a, b) for d in range(3)]
+                            blk[x, y] += ca * cb * (T1[0] * S1[1] * S1[2] + S1[0] * T1[1] * S1[2]
+                                                    + S1[0] * S1[1] * T1[2])
+            K[off[i]:off[i] + si.nfunc, off[j]:off[j] + sj.nfunc] = blk
+    return K
+
+
+def nuclear_attraction(shells, atoms) -> np.ndarray:
+    """V_{mu nu} = -sum_A Z_A <mu| 1/|r - R_A| |nu> (AOMultipole::FillPotential with the
+    nuclear charges, aomultipole.cc:2095-2105): Hermite expansion with R_{tuv}(p, P - C)."""
+    off, n = _offsets(shells)
+    V = np.zeros((n, n))
+    for i, si in enumerate(shells):
+        for j, sj in enumerate(shells):
+            blk = np.zeros((si.nfunc, sj.nfunc))
+            for a, ca in zip(si.exps, si.coefs):
+                for b, cb in zip(sj.exps, sj.coefs):
+                    p = a + b
+                    Pc = (a * si.center + b * sj.center) / p
+                    h = _herm_pair(si.l, si.center, a, sj.l, sj.center, b)
+                    for sym, Cpos in atoms:
+                        X, Y, Z = Pc - Cpos
+                        cache: dict = {}
+                        for x in range(si.nfunc):
+                            for y in range(sj.nfunc):
+                                acc = 0.0
+                                for (t, u, v), e in h[(x, y)].items():
+                                    acc += e * _R(t, u, v, 0, p, X, Y, Z, cache)
+                                blk[x, y] -= NUCLEAR_CHARGE[sym] * ca * cb * 2.0 * math.pi / p * acc
+            V[off[i]:off[i] + si.nfunc, off[j]:off[j] + sj.nfunc] = blk
+    return V
+
+
+def core_hamiltonian(xyz_path: str, basis_path: str) -> np.ndarray:
+    """Kinetic plus nuclear attraction, the matrix test_ppm.cc diagonalises for its MOs."""
+    shells = build_shells(xyz_path, basis_path)
+    return kinetic_matrix(shells) + nuclear_attraction(shells, read_xyz(xyz_path))
+
+
 @lru_cache(maxsize=8)
 def ao_integrals(xyz_path: str, basis_path: str, aux_basis_path: str | None = None):
     """Return (S_dft, S_aux, V_aux, T3c) for a molecule and basis (aux defaults to dft)."""

@@ -212,6 +212,31 @@ class BSEOperatorOracle:
         return res
 
 
+def rpa_h2p(M: np.ndarray, energies: np.ndarray, homo: int, rpamin: int, rpamax: int):
+    """RPA::Diagonalize_H2p / Calculate_H2p_AmB / Calculate_H2p_ApB / Diagonalize_H2p_C
+    (xtp/src/libxtp/gwbse/rpa.cc:204-345, closed shell, no QSGW rotation):
+    (A-B)_{vc} = e_c - e_v, A+B = diag(A-B) + 4 V V^T with V_{I(v,c),P} = M^P_{vc};
+    C = (A-B)^1/2 (A+B) (A-B)^1/2 = U diag(omega^2) U^T;
+    E_c = -1/4 (tr(A+B) + sum(A-B)) + 1/2 sum(omega); (X+Y)_s = omega_s^-1/2 (A-B)^1/2 U_s.
+    Returns (omega ascending, XpY, E_c)."""
+    lumo = homo + 1
+    n_occ = lumo - rpamin
+    n_unocc = rpamax - lumo + 1
+    e = np.asarray(energies, dtype=np.float64)
+    amb = (e[n_occ:n_occ + n_unocc][None, :] - e[:n_occ][:, None]).reshape(-1)
+    V = M[:n_occ, n_occ:n_occ + n_unocc, :].reshape(n_occ * n_unocc, -1)
+    apb = 4.0 * (V @ V.T) + np.diag(amb)
+    erpa = -0.25 * (np.trace(apb) + amb.sum())
+    r = np.sqrt(amb)
+    lam, U = np.linalg.eigh(r[:, None] * apb * r[None, :])
+    if lam.min() <= 0.0:
+        raise RuntimeError("Detected non-positive eigenvalue.")
+    omega = np.sqrt(lam)
+    erpa += 0.5 * omega.sum()
+    xpy = r[:, None] * U / np.sqrt(omega)[None, :]
+    return omega, xpy, erpa
+
+
 def bse_dense(cqp, cx, cd, cd2, eps_inv, M, Hqp, homo, rpamin, vmin, cmax) -> np.ndarray:
     """Explicit BSE matrix from the closed-form matrix elements the reference's
     matmul contracts (bse_operator.cc:61-116): H[(v,c),(v',c')] =

@@ -465,6 +465,56 @@ def test_sigma_ppm_fixtures_gpu(ctx, ints):  # test_sigma_ppm.cc:44-121
     assert abs((s[1] - s[0]) / 2e-5 - d[2]) < 1e-5 * max(1.0, abs(d[2]))
 
 
+
+def test_ppm_known_answer_gpu(ctx):  # test_ppm.cc:36-90
+    """Plasmon-pole frequencies and weights on the device for CH4 / 3-21G with the core-Hamiltonian
+    MOs and energies: the test's values at isApprox 1e-4 and the oracle to 1e-8."""
+    xyz, bas = os.path.join(G, "ppm/molecule.xyz"), os.path.join(G, "ppm/3-21G.xml")
+    e, Cm = np.linalg.eigh(I.core_hamiltonian(xyz, bas))
+    S, Saux, V, T = I.ao_integrals(xyz, bas)
+    t = px.TCMatrix_gwbse(ctx).Initialize(17, 0, 16, 0, 16)
+    t.Fill(T, Cm, X.pseudo_invsqrt_gwbse(V, Saux))
+    M = t.get_tensor()
+    rpa = px.RPA(t)
+    rpa.configure(4, 0, 16)
+    rpa.setRPAInputEnergies(e)
+    sig = px.Sigma_PPM(t, rpa).configure(4, 0, 16, 0, 16, 1e-3)
+    sig.PrepareScreening()
+    freq = [19.4503, 12.2429, 10.2167, 10.2167, 10.2167, 17.0832, 12.7117, 12.7117, 12.7117, 10.9455, 10.9455,
+            10.9455, 11.1861, 9.60843, 9.60843, 9.60843, 9.61518]
+    w = [3.59422e-05, 0.00121795, 0.00343632, 0.00343632, 0.00343632, 0.00394659, 0.012066, 0.012066, 0.012066,
+         0.0241118, 0.0241118, 0.0241118, 0.0286786, 0.11036, 0.11036, 0.11036, 0.191732]
+    assert mmio.isapprox(sig.ppm_freq, np.array(freq), 1e-4)
+    assert mmio.isapprox(sig.ppm_weight, np.array(w), 1e-4)
+    phi, wo, fo = X.ppm_construct(M, e, 4, 0, 16)
+    # the first pole carries a weight of 3.6e-5: its frequency amplifies rounding in eps ~1/w
+    assert np.abs(sig.ppm_freq - fo).max() < 1e-6
+    assert np.abs(sig.ppm_weight - wo).max() < 1e-10
+
+
+def test_rpa_h2p_fixture_gpu(ctx, ints):  # test_rpa_h2p.cc:41-110
+    """RPA two-particle Hamiltonian on the device: the 60 poles (isApprox 1e-4), |diag(X+Y)|
+    (1e-4) and E_c (1e-6 relative) of the reference's test, and the oracle to 1e-10."""
+    from test_oracle_golden import H2P_ERPA, H2P_OMEGA, H2P_XPY_DIAG
+    t = ch4_tensor(ctx, ints, "rpa/eigenvectors.mm")
+    e = mmio.read_vector(os.path.join(G, "rpa/eigenvals.mm"))
+    rpa = px.RPA(t)
+    rpa.configure(4, 0, 16)
+    rpa.setRPAInputEnergies(e)
+    sol = rpa.Diagonalize_H2p()
+    assert abs(sol.ERPA_correlation - H2P_ERPA) <= 1e-6 * abs(H2P_ERPA)
+    assert mmio.isapprox(sol.omega, H2P_OMEGA, 1e-4)
+    assert mmio.isapprox(np.abs(np.diag(sol.XpY)), np.abs(H2P_XPY_DIAG), 1e-4)
+    om, xpy, er = X.rpa_h2p(t.get_tensor(), e, 4, 0, 16)
+    assert np.abs(sol.omega - om).max() < 1e-10 and abs(sol.ERPA_correlation - er) < 1e-12
+    # X+Y columns agree through the projector on each group of poles closer than 1e-5 (the
+    # degenerate and near-degenerate pairs; a pole split by g moves its vector by ~rounding / g)
+    for s0 in range(60):
+        grp = np.where(np.abs(om - om[s0]) < 1e-5)[0]
+        Pd = sol.XpY[:, grp] @ np.linalg.pinv(sol.XpY[:, grp])
+        Po = xpy[:, grp] @ np.linalg.pinv(xpy[:, grp])
+        assert np.abs(Pd - Po).max() < 1e-7
+
 def test_gw_fixture_gpu(ctx, ints):  # test_gw.cc:111-190
     t = ch4_tensor(ctx, ints, "gw/mo_eigenvectors.mm")
     vxc = mmio.read_matrix(os.path.join(G, "gw/vxc.mm"))

@@ -341,3 +341,94 @@ def test_dynamical_screening_fixtures(ints, bse_chain, name, c, nroots, full, to
     assert np.array_equal(X.perturbative_dynamical_screening(M, e, 4, 0, 16, Hqp, 0, 16, ev, Xv, Yv, max_iter=0), ev)
     zero = X.perturbative_dynamical_screening(M, e, 4, 0, 16, Hqp, 0, 16, np.zeros_like(ev), Xv, Yv)
     assert np.abs(zero).max() < 1e-12
+
+
+# test_ppm.cc:66-73: plasmon-pole frequencies and weights of CH4 / 3-21G with the MOs of the
+# core Hamiltonian (kinetic + nuclear attraction), RPA energies = its eigenvalues
+PPM_FREQ = np.array([19.4503, 12.2429, 10.2167, 10.2167, 10.2167, 17.0832, 12.7117, 12.7117, 12.7117, 10.9455,
+                     10.9455, 10.9455, 11.1861, 9.60843, 9.60843, 9.60843, 9.61518])
+PPM_W = np.array([3.59422e-05, 0.00121795, 0.00343632, 0.00343632, 0.00343632, 0.00394659, 0.012066, 0.012066,
+                  0.012066, 0.0241118, 0.0241118, 0.0241118, 0.0286786, 0.11036, 0.11036, 0.11036, 0.191732])
+
+
+def ppm_inputs():
+    xyz, bas = os.path.join(G, "ppm/molecule.xyz"), os.path.join(G, "ppm/3-21G.xml")
+    e, C = np.linalg.eigh(I.core_hamiltonian(xyz, bas))
+    return xyz, bas, e, C
+
+
+def test_ppm_known_answer():  # test_ppm.cc:36-90
+    xyz, bas, e, C = ppm_inputs()
+    S, Saux, V, T = I.ao_integrals(xyz, bas)
+    M = X.fill_mmn(T, C, 0, 16, 0, 16, Saux, V)
+    phi, w, f = X.ppm_construct(M, e, 4, 0, 16)
+    assert mmio.isapprox(f, PPM_FREQ, 1e-4)
+    assert mmio.isapprox(w, PPM_W, 1e-4)
+
+
+def test_one_electron_integrals_closed_forms():
+    """Kinetic and nuclear attraction for a single normalised s primitive exp(-a r^2) at a nucleus
+    of charge Z: T = 3a/2, V = -2 Z sqrt(2a/pi); both matrices symmetric; the hydrogen 3-21G
+    core Hamiltonian's lowest eigenvalue lies above the exact -0.5 Ha (variational bound)."""
+    sh = I.Shell(0, np.zeros(3), np.array([0.7]), np.array([I._prim_norm(0.7, 0)]))
+    assert abs(I.kinetic_matrix([sh])[0, 0] - 1.5 * 0.7) < 1e-13
+    assert abs(I.nuclear_attraction([sh], [("C", np.zeros(3))])[0, 0] + 2 * 6.0 * np.sqrt(2 * 0.7 / np.pi)) < 1e-12
+    xyz = os.path.join(G, "ppm/molecule.xyz")
+    shells = I.build_shells(xyz, os.path.join(G, "ppm/3-21G.xml"))
+    K, Vn = I.kinetic_matrix(shells), I.nuclear_attraction(shells, I.read_xyz(xyz))
+    assert np.abs(K - K.T).max() < 1e-13 and np.abs(Vn - Vn.T).max() < 1e-12
+    hshell = [s for s in shells if np.allclose(s.center, I.read_xyz(xyz)[1][1])]
+    Sh = I.overlap_matrix(hshell)
+    Hh = I.kinetic_matrix(hshell) + I.nuclear_attraction(hshell, [("H", hshell[0].center)])
+    lam = np.linalg.eigvalsh(np.linalg.solve(np.linalg.cholesky(Sh), np.linalg.solve(np.linalg.cholesky(Sh), Hh).T))
+    assert -0.5 < lam[0] < -0.49
+
+
+# test_rpa_h2p.cc:63-88: the 60 RPA poles, the diagonal of X+Y (absolute values) and E_c
+H2P_OMEGA = np.array([
+    0.104192, 0.104192, 0.187814, 0.559693, 0.559693, 0.572575, 0.577988, 0.577989, 0.579088, 0.618403, 0.618403,
+    0.67005, 0.678538, 0.678538, 0.722771, 1.10797, 1.41413, 1.41413, 1.58866, 1.60381, 1.60381, 1.64709, 1.87331,
+    1.87331, 1.88646, 1.8926, 1.89268, 1.89268, 1.933, 1.933, 2.01832, 2.40974, 2.42192, 2.42192, 2.46371, 2.85829,
+    2.8853, 2.8853, 2.90367, 2.90367, 2.92541, 2.94702, 3.3382, 3.3382, 3.35102, 3.3566, 3.3566, 3.35835, 3.39617,
+    3.39617, 4.22882, 4.71607, 4.72233, 4.72233, 4.76567, 16.5917, 17.0793, 17.093, 17.093, 17.1377])
+H2P_XPY_DIAG = np.array([
+    0.00898067, -0.00898068, 0.00228621, -7.40059e-10, -0.000447591, -7.21786e-10, 2.86571e-09, 1.16917e-08,
+    5.18349e-09, 0.000124051, 1.89797e-10, -1.549e-05, -0.00513947, -0.00513956, -7.71176e-08, -2.09605e-08,
+    0.00412762, 0.00412738, 6.253e-09, 3.93273e-05, 2.09406e-05, -0.000532391, 3.70347e-05, 2.0857e-06,
+    -6.13653e-10, 0.000723575, 3.41096e-05, 0.00942648, 0.0497694, 0.0497693, -1.28726e-07, 3.68417e-09,
+    -7.00924e-06, 7.01603e-06, -1.36252e-09, -1.08353e-09, 0.000549202, 0.000549203, -6.22315e-09, -3.83321e-08,
+    -2.4032e-08, -2.19679e-09, 5.76794e-10, -1.46987e-08, -1.27183e-08, -0.138355, -6.49432e-09, -2.71175e-05,
+    2.69024e-05, -2.69024e-05, 0.000155952, 0.000311876, -0.0002498, -0.000249804, 0.000816289, 1.05559e-06,
+    1.84996e-12, 5.76753e-06, -9.90455e-11, -0.000800706])
+H2P_ERPA = -0.0587973
+
+
+def test_rpa_h2p_fixture(ints):  # test_rpa_h2p.cc:41-110
+    M = mmn_for(ints, "rpa/eigenvectors.mm", 0, 16, 0, 16)
+    e = mmio.read_vector(os.path.join(G, "rpa/eigenvals.mm"))
+    om, xpy, er = X.rpa_h2p(M, e, 4, 0, 16)
+    assert abs(er - H2P_ERPA) <= 1e-6 * abs(H2P_ERPA)  # BOOST_CHECK_CLOSE 1e-4 percent
+    assert mmio.isapprox(om, H2P_OMEGA, 1e-4)
+    assert mmio.isapprox(np.abs(np.diag(xpy)), np.abs(H2P_XPY_DIAG), 1e-4)
+
+
+def test_rpa_h2p_closed_forms():
+    """M = 0: the poles are the sorted e_c - e_v and E_c = 0. Otherwise the poles are the positive
+    eigenvalues of the RPA matrix [[A, B], [-B, -A]] with A = diag(D) + 2 V V^T, B = 2 V V^T, and
+    X+Y solves (A-B)(A+B)(X+Y) = Omega^2 (X+Y)."""
+    rng = np.random.default_rng(3)
+    e = np.sort(rng.uniform(-1, 1, 9))
+    om, xpy, er = X.rpa_h2p(np.zeros((9, 9, 5)), e, 3, 0, 8)
+    D = (e[4:][None, :] - e[:4][:, None]).ravel()
+    assert np.allclose(om, np.sort(D), atol=1e-14) and abs(er) < 1e-14
+    M = rng.normal(scale=0.1, size=(9, 9, 5))
+    M = 0.5 * (M + M.transpose(1, 0, 2))
+    om, xpy, er = X.rpa_h2p(M, e, 3, 0, 8)
+    V = M[:4, 4:, :].reshape(20, 5)
+    A = np.diag(D) + 2 * V @ V.T
+    B = 2 * V @ V.T
+    w = np.linalg.eigvals(np.block([[A, B], [-B, -A]])).real
+    assert np.allclose(om, np.sort(w[w > 0]), atol=1e-12)
+    # (A-B)(A+B)(X+Y) = Omega^2 (X+Y), normalised as (X+Y)^T (A-B)^-1 (X+Y) = 1/Omega
+    assert np.abs((A - B) @ (A + B) @ xpy - xpy * om ** 2).max() < 1e-12
+    assert np.allclose(np.einsum("is,i,is->s", xpy, 1.0 / D, xpy), 1.0 / om, atol=1e-12)

@@ -110,6 +110,7 @@ SIGNATURES = {
     "xtpgpu_epsbatch_destroy": (C.c_int, [_vp]),
     "xtpgpu_bse_setup_screening": (C.c_int, [_vp, _vp, _dp, _i64, _i64, _i64, C.c_double, _dp]),
     "xtpgpu_sigma_x": (C.c_int, [_vp, _vp, _i64, _i64, _i64, _i64, _dp]),
+    "xtpgpu_rpa_h2p": (C.c_int, [_vp, _vp, _dp, _i64, _i64, _i64, _dp, _dp, _dp]),
     "xtpgpu_ppm_prepare": (C.c_int, [_vp, _vp, _dp, _i64, _i64, _i64, _dp, _dp]),
     "xtpgpu_sigma_ppm_create": (C.c_int, [_vp, _vp, _dp, _dp, _dp, _i64, _i64, _i64, C.c_double,
                                           C.POINTER(_vp)]),
@@ -363,6 +364,25 @@ class RPA:
         if isinstance(frequency, complex):
             return self.calculate_epsilon(EPS_COMPLEX, [frequency], precision)[:, :, 0]
         return self.calculate_epsilon(EPS_REAL, [frequency], precision)[:, :, 0]
+
+    def Diagonalize_H2p(self, want_vectors=True) -> "rpa_eigensolution":
+        """RPA::Diagonalize_H2p (rpa.cc:204-258) on the device: poles omega, X+Y vectors and the
+        RPA correlation energy."""
+        n = (self.homo + 1 - self.rpamin) * (self.rpamax - self.homo)
+        om = np.zeros(n)
+        xpy = np.zeros((n, n), order="F") if want_vectors else None
+        er = np.zeros(1)
+        _check(lib().xtpgpu_rpa_h2p(self.ctx.h, self.Mmn.h, _dptr(self.energies), self.homo, self.rpamin,
+                                    self.rpamax, _dptr(om), _dptr(xpy) if want_vectors else None, _dptr(er)))
+        return rpa_eigensolution(om, xpy, float(er[0]))
+
+
+@dataclass
+class rpa_eigensolution:
+    """RPA::rpa_eigensolution (rpa.h:62-66)."""
+    omega: np.ndarray
+    XpY: np.ndarray | None
+    ERPA_correlation: float
 
 
 @dataclass
