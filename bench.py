@@ -243,7 +243,8 @@ def run_adjoint(args):
     lowest `--adj-roots` with respect to the QP energies (all levels) and the transition block
     M^P_{vc} (all P, v, c) at fixed screening, checked against central finite differences
     (h = `--fd-h` Ha) on `--fd-levels` seeded QP levels and `--fd-mmn` seeded Mmn entries; each
-    finite difference re-solves the perturbed problem twice in FP64."""
+    finite difference re-solves the perturbed problem twice in FP64, started from the converged
+    vectors and converged to `--fd-tol`."""
     from votca_amd import pyxtp as px
     o, v, P = args.nocc, args.nvirt, args.naux
     n = o + v
@@ -260,18 +261,20 @@ def run_adjoint(args):
     epsinv = np.where(lam > 1e-8, 1.0 / lam, 0.0)
     opt = px.BSEOperator_Options(o - 1, 0, 0, 0, n - 1)
 
-    def solve(Hqp):
+    def solve(Hqp, guess=None, tol=None):
         A = px.SingletOperator_TDA(epsinv, Mmn, Hqp).configure(opt)
         B = px.SingletOperator_BTDA_B(epsinv, Mmn, Hqp).configure(opt)
         ds = px.DavidsonSolver()
-        ds.set_tolerance(args.tol)
+        ds.set_tolerance(args.tol if tol is None else tol)
         ds.set_correction("DPR")
         ds.set_size_update("safe")
         ds.set_max_search_space(args.adj_space or 10 * args.roots)
         ds.set_iter_max(args.iter_max)
         t0 = time.perf_counter()
-        ev, Xv, Yv = ds.solve_full(A, B, args.roots)
+        ev, Xv, Yv = ds.solve_full(A, B, args.roots, initial_guess=guess)
         wall = time.perf_counter() - t0
+        if guess is not None:
+            log(f"[adjoint] perturbed solve from the converged vectors: {wall:.2f} s, {ds.info.iterations} it")
         if not ds.converged():
             raise RuntimeError(f"full BSE did not converge (residual {ds.info.max_residual:.2e})")
         return A, B, ev, Xv, Yv, wall, ds.info
@@ -308,6 +311,8 @@ def run_adjoint(args):
     log(f"[adjoint] d/d(e_QP) and d/dM for {na} roots: {t_adj*1e3:.1f} ms ({na*P*N*8/1e9:.2f} GB of dOmega/dM)")
     ev_stat = stationary(A, B, Xv, Yv)
     A.close(), B.close()
+    # the perturbed problems (h ~ 1e-5) start from the converged [X; Y] (initial-guess overload)
+    warm = np.vstack([Xv, Yv])
     rng = np.random.default_rng(args.seed + 1)
     h = args.fd_h
     checks = []
@@ -316,7 +321,7 @@ def run_adjoint(args):
         for sgn in (1.0, -1.0):
             Hp = H0.copy()
             Hp[lvl, lvl] += sgn * h
-            a_, b_, evp, xp_, yp_, *_ = solve(Hp)
+            a_, b_, evp, xp_, yp_, *_ = solve(Hp, warm, args.fd_tol)
             w.append(stationary(a_, b_, xp_, yp_))
             a_.close(), b_.close()
         fd = (w[0] - w[1]) / (2 * h)
@@ -330,7 +335,7 @@ def run_adjoint(args):
             m = col.copy()
             m[p_] += sgn * h
             Mmn.set_block(v_, o + c_, m.reshape(P, 1, 1), symmetric=True)
-            a_, b_, evp, xp_, yp_, *_ = solve(H0)
+            a_, b_, evp, xp_, yp_, *_ = solve(H0, warm, args.fd_tol)
             w.append(stationary(a_, b_, xp_, yp_))
             a_.close(), b_.close()
         Mmn.set_block(v_, o + c_, col.reshape(P, 1, 1), symmetric=True)
@@ -355,7 +360,7 @@ def run_adjoint(args):
         log(f"[adjoint] {kind} {where}: max rel err {rel:.2e}")
     print(json.dumps({"workload": f"full BSE + adjoint, n_occ={o} n_virt={v} n_aux={P}, {args.roots} roots",
                       "solve_s": wall, "iterations": info.iterations, "adjoint_s": t_adj, "adjoint_roots": na,
-                      "dOmega_dM_bytes": na * P * N * 8, "fd_h": h, "max_rel_err": worst,
+                      "dOmega_dM_bytes": na * P * N * 8, "fd_h": h, "fd_tol": args.fd_tol, "max_rel_err": worst,
                       "normwise_rel_err": normwise, "fd_rounding_floor": noise, "checks": rows}))
 
 
@@ -387,6 +392,8 @@ def main():
     ap.add_argument("--fd-levels", type=int, default=3)
     ap.add_argument("--fd-mmn", type=int, default=3)
     ap.add_argument("--fd-h", type=float, default=1e-5)
+    ap.add_argument("--fd-tol", type=float, default=1e-9,
+                    help="residual tolerance of the perturbed full-BSE solves of the central differences")
     ap.add_argument("--adj-space", type=int, default=0, help="full-BSE max search space (default 10 * roots)")
     ap.add_argument("--cols", type=int, default=30)
     ap.add_argument("--copy-calibration", action="store_true",

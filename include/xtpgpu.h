@@ -268,6 +268,17 @@ int xtpgpu_davidson_solve(xtpgpu_bseop op, int64_t neigen, const xtpgpu_davidson
 int xtpgpu_davidson_solve_full(xtpgpu_bseop A, xtpgpu_bseop B, int64_t neigen,
                                const xtpgpu_davidson_opts* opts, double* evals, double* X,
                                double* Y, int64_t ldv, xtpgpu_davidson_info* info);
+/* The initial-guess overload of DavidsonSolver::solve (davidsonsolver.h:123-152) for the full BSE:
+ * the nguess (>= neigen) columns of guess (2*size x nguess, column-major, leading dimension
+ * ldg >= 2*size, [X; Y] stacked) are scaled to unit length, orthonormalized and start the search
+ * space (the reference's Gram-Schmidt assumes a unit first column, which its ranked guess has;
+ * a guess narrower than the size update simply yields fewer correction vectors); the restart size
+ * is min(nguess, max search space). Used to continue from converged eigenvectors of a nearby
+ * problem (e.g. a perturbed Hqp). */
+int xtpgpu_davidson_solve_full_guess(xtpgpu_bseop A, xtpgpu_bseop B, int64_t neigen,
+                                     const xtpgpu_davidson_opts* opts, const double* guess, int64_t ldg,
+                                     int64_t nguess, double* evals, double* X, double* Y, int64_t ldv,
+                                     xtpgpu_davidson_info* info);
 
 /* ---------------------------------------------------------------- batched segments */
 /* eps(i w) for many segments of uneven size (the per-segment calculate_epsilon_i of

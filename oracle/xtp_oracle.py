@@ -392,12 +392,14 @@ class DavidsonOracle:
         return self.eigenvalues, self.eigenvectors
 
 
-    def solve_ham(self, matmul, diag_a, diag_b, neigen):
+    def solve_ham(self, matmul, diag_a, diag_b, neigen, guess=None):
         """HAM matrix type as called by BSE::Solve_nonhermitian_Davidson (bse.cc:320-358): guess
         from BuildFullBSEXRankedInitialGuess (bse_initialization.h:47-93), diagonal [Ad; -Ad]
         (bseoperator_btda.h:85-91), projections T = V^T H V and B = V^T H H V (davidsonsolver.h:
         239-280), harmonic Ritz pairs (davidsonsolver.cc:241-332) and the QR restart
-        (davidsonsolver.cc:499-518). `matmul` applies H = [[A, B], [-B, -A]] to 2N x k blocks."""
+        (davidsonsolver.cc:499-518). `matmul` applies H = [[A, B], [-B, -A]] to 2N x k blocks.
+        `guess` (2N x k) replaces the ranked guess: the initial-guess overload of
+        DavidsonSolver::solve (davidsonsolver.h:123-152), restart size min(k, max space)."""
         import scipy.linalg as sla
         diag_a = np.asarray(diag_a, dtype=np.float64)
         diag_b = np.asarray(diag_b, dtype=np.float64)
@@ -408,12 +410,17 @@ class DavidsonOracle:
             self.max_search_space = neigen * 5
         if self.max_search_space > n:
             self.max_search_space = n
-        nguess = min(nh, max(4 * neigen, 8))
-        omega = np.sqrt(np.maximum(0.0, (diag_a - diag_b) * (diag_a + diag_b)))
-        rank = sorted(range(nh), key=lambda i: (omega[i], diag_a[i]))
-        V = np.zeros((n, nguess))
-        for j in range(nguess):
-            V[rank[j], j] = 1.0
+        if guess is None:
+            nguess = min(nh, max(4 * neigen, 8))
+            omega = np.sqrt(np.maximum(0.0, (diag_a - diag_b) * (diag_a + diag_b)))
+            rank = sorted(range(nh), key=lambda i: (omega[i], diag_a[i]))
+            V = np.zeros((n, nguess))
+            for j in range(nguess):
+                V[rank[j], j] = 1.0
+        else:  # unit columns: gramschmidt (as the reference's) never normalizes column 0
+            V = np.array(guess, dtype=np.float64)
+            V = V / np.linalg.norm(V, axis=0)
+            nguess = V.shape[1]
         V = self.gramschmidt(V, 0)
         restart_size = min(nguess, self.max_search_space)
         size_update = self.size_update(neigen)
@@ -451,6 +458,7 @@ class DavidsonOracle:
             res = AV @ U - q * lam[None, :]
             rn = np.linalg.norm(res, axis=0)
             self.history.append(float(rn[:neigen].max()))
+            # a guess narrower than the size update leaves fewer pairs than size_update
             conv = rn[:size_update] < self.tol
             if np.all(conv[:neigen]):
                 self.info_converged = True
@@ -464,7 +472,7 @@ class DavidsonOracle:
                         q[:, i] = 0.0
                 break
             newv = []
-            for j in range(size_update):
+            for j in range(len(conv)):
                 if conv[j]:
                     continue
                 if self.correction == "DPR":

@@ -185,6 +185,36 @@ def test_bse_energy_fixtures_gpu(ctx, ints, name, psi, c, offdiag, nroots, full)
 
 
 
+
+def test_full_bse_initial_guess_gpu(ctx, ints):
+    """The initial-guess overload (davidsonsolver.h:123-152) on the full BSE of the reference's test:
+    started from converged [X; Y] it converges at once to the same roots; a guess with fewer
+    columns than roots or the wrong row count is refused with the reference's messages."""
+    S, Saux, V, T = ints
+    C = mmio.read_matrix(os.path.join(G, "bse/MOs.mm"))
+    Hqp = mmio.read_matrix(os.path.join(G, "bse/Hqp.mm"))
+    t = px.TCMatrix_gwbse(ctx).Initialize(17, 0, 16, 0, 16)
+    t.Fill(T, C, X.pseudo_invsqrt_gwbse(V, Saux))
+    epsinv = px.setup_screening(t, np.diag(Hqp).copy(), 4, 0, 16)
+    opt = px.BSEOperator_Options(4, 0, 0, 0, 16)
+    A = px.BSE_OPERATOR(1, 2, 1, 0, epsinv, t, Hqp).configure(opt)
+    B = px.BSE_OPERATOR(0, 2, 0, 1, epsinv, t, Hqp).configure(opt)
+    ds = px.DavidsonSolver()
+    ds.set_tolerance("lapack")
+    ds.set_max_search_space(30)
+    ev, Xv, Yv = ds.solve_full(A, B, 3)
+    assert ds.converged()
+    ev2, X2, Y2 = ds.solve_full(A, B, 3, initial_guess=np.vstack([Xv, Yv]))
+    assert ds.converged() and ds.num_iterations() <= 1
+    assert np.abs(ev2 - ev).max() < 1e-12
+    # the three roots are degenerate (T2): compare the spanned subspaces
+    Z1, Z2 = np.vstack([Xv, Yv]), np.vstack([X2, Y2])
+    assert np.abs(Z1 @ np.linalg.pinv(Z1) - Z2 @ np.linalg.pinv(Z2)).max() < 1e-7
+    with pytest.raises(px.XtpGpuError, match="fewer columns"):
+        ds.solve_full(A, B, 3, initial_guess=np.vstack([Xv, Yv])[:, :2])
+    with pytest.raises(px.XtpGpuError, match="wrong number of rows"):
+        ds.solve_full(A, B, 3, initial_guess=Xv)
+
 DYN_CASES = [  # test_bse.cc:177-191, 268-282, 326-340: (fixture, operator, roots, full BSE, isApprox tolerance)
     ("singlets_dynamic_TDA", (1, 2, 1, 0), 3, False, 0.005),
     ("triplets_dynamic_TDA", (1, 0, 1, 0), 1, False, 0.001),

@@ -306,6 +306,11 @@ def test_ham_davidson_oracle_fixture(bse_chain):
     assert np.abs(np.linalg.norm(R.T @ unit_columns(Xv), axis=0) - 1).max() < 1e-5
     RA = unit_columns(mmio.read_matrix(os.path.join(G, "bse/singlets_psi_AR_btda.mm")))
     assert np.abs(np.linalg.norm(RA.T @ unit_columns(Yv), axis=0) - 1).max() < 1e-5
+    # the initial-guess overload started from the converged [X; Y] stops at iteration 0
+    d2 = X.DavidsonOracle(tol="lapack", correction="DPR", update="safe", iter_max=50, max_search_space=30)
+    ev2, X2, Y2 = d2.solve_ham(lambda v: H @ v, np.diag(A), np.diag(B), 3, guess=Z)
+    assert d2.info_converged and d2.num_iterations == 0
+    assert np.abs(ev2 - ev).max() < 1e-12
 
 
 DYN_CASES = [  # test_bse.cc:177-191, 268-282, 326-340: (fixture, operator, roots, full BSE, isApprox tolerance)

@@ -95,6 +95,8 @@ SIGNATURES = {
                                         C.POINTER(DavidsonInfo)]),
     "xtpgpu_davidson_solve_full": (C.c_int, [_vp, _vp, _i64, C.POINTER(DavidsonOpts), _dp, _dp, _dp, _i64,
                                              C.POINTER(DavidsonInfo)]),
+    "xtpgpu_davidson_solve_full_guess": (C.c_int, [_vp, _vp, _i64, C.POINTER(DavidsonOpts), _dp, _i64, _i64, _dp,
+                                                   _dp, _dp, _i64, C.POINTER(DavidsonInfo)]),
     "xtpgpu_bse_adjoint": (C.c_int, [_vp, _vp, _i64, _dp, _dp, _i64, _dp, _dp]),
     "xtpgpu_bse_adjoint_device": (C.c_int, [_vp, _vp, _i64, _dp, _dp, _i64, _dp, _vp]),
     "xtpgpu_sigma_cda_create": (C.c_int, [_vp, _vp, _dp, _i64, _i64, _i64, _i64, _i64, C.c_double, C.c_double,
@@ -550,14 +552,22 @@ class DavidsonSolver:
             _check(rc)
         return ev, vec
 
-    def solve_full(self, A: BSE_OPERATOR, B: BSE_OPERATOR, neigen: int, want_vectors=True):
-        """Full BSE with the HamiltonianOperator [[A,B],[-B,-A]] (BSE::Solve_nonhermitian_Davidson)."""
+    def solve_full(self, A: BSE_OPERATOR, B: BSE_OPERATOR, neigen: int, want_vectors=True, initial_guess=None):
+        """Full BSE with the HamiltonianOperator [[A,B],[-B,-A]] (BSE::Solve_nonhermitian_Davidson);
+        `initial_guess` (2N x k, [X; Y]) selects the initial-guess overload of DavidsonSolver::solve."""
         ev = np.zeros(neigen)
         Xv = np.zeros((A.size, neigen), order="F") if want_vectors else None
         Yv = np.zeros((A.size, neigen), order="F") if want_vectors else None
-        rc = lib().xtpgpu_davidson_solve_full(A.h, B.h, neigen, C.byref(self.opts), _dptr(ev),
-                                              _dptr(Xv) if want_vectors else None,
-                                              _dptr(Yv) if want_vectors else None, A.size, C.byref(self.info))
+        if initial_guess is not None:
+            g = _f64(initial_guess)
+            rc = lib().xtpgpu_davidson_solve_full_guess(A.h, B.h, neigen, C.byref(self.opts), _dptr(g), g.shape[0],
+                                                        g.shape[1], _dptr(ev), _dptr(Xv) if want_vectors else None,
+                                                        _dptr(Yv) if want_vectors else None, A.size,
+                                                        C.byref(self.info))
+        else:
+            rc = lib().xtpgpu_davidson_solve_full(A.h, B.h, neigen, C.byref(self.opts), _dptr(ev),
+                                                  _dptr(Xv) if want_vectors else None,
+                                                  _dptr(Yv) if want_vectors else None, A.size, C.byref(self.info))
         self._eval, self._evec, self._evec2 = ev, Xv, Yv
         self._converged = rc == 0
         if rc not in (0, -4):
