@@ -40,7 +40,7 @@ PEAK = {"fp64": 78.6e12, "bf16": 2.5e15, "hbm": 8.0e12}
 # for the split form (two output planes), 573 for plain BF16.
 HD_STAGE1_TRAFFIC = {"bf16x3": {"read": 4.078 * 2.0 * 2**30, "write": 3.311 * 1.0 * 2**30,
                                 "launch": "30-column H X stage-1 launch, split BF16 (P-chunk 286)"},
-                     "bf16": {"read": 2.254 * 2.0 * 2**30, "write": 2.908 * 1.0 * 2**30,
+                     "bf16": {"read": 1.958 * 2.0 * 2**30, "write": 3.074 * 1.0 * 2**30,
                               "launch": "30-column H X stage-1 launch, plain BF16 (P-chunk 573)"}}
 
 
@@ -224,6 +224,11 @@ def run_matvec(args):
         out[prec] = {"wall_ms": wall * 1e3,
                      **{f"{name}_tflops": t["flops"] / max(t["ms"] / 1e3, 1e-30) / 1e12 for name, t in tm.items()},
                      **{f"{name}_ms": t["ms"] / max(1, args.steps) for name, t in tm.items()}}
+        y = Y.download((N, k))
+        if prec == "fp64":
+            y64 = y
+        else:  # the mixed product against the FP64 one (Frobenius norm)
+            out[prec]["rel_err_vs_fp64"] = float(np.linalg.norm(y - y64) / np.linalg.norm(y64))
         log(f"[matvec] {prec}: {wall*1e3:.2f} ms per H X ({k} cols); " +
             ", ".join(f"{nm} {t['ms']/args.steps:.2f} ms {t['flops']/max(t['ms']/1e3,1e-30)/1e12:.1f} TF"
                       for nm, t in tm.items()))
@@ -536,12 +541,13 @@ def main():
     kt = tm["ms"] / 1e3 / max(1, tm["launches"])
     ach = tm["flops"] / max(tm["ms"] / 1e3, 1e-30)
     issue = 3.0 if dom == "bf16x3" else 1.0
+    choice = os.environ.get("XTPGPU_BF16_KERNEL", "")
     kname = {"staged32": "gemm_bf16_kernel<128,128,..,EpiStage1,32>", "pipe": "gemm_bf16x3_pipe_kernel<..,EpiStage1>",
-             "big": "gemm_bf16_kernel<256,256,..,EpiStage1,32>", "tall": "gemm_bf16_kernel<256,128,..,EpiStage1,64,512>"
-             }.get(os.environ.get("XTPGPU_BF16_KERNEL", ""), "gemm_bf16_kernel<128,128,..,EpiStage1,64>")
+             "big": "gemm_bf16_kernel<256,256,..,EpiStage1,32>", "tall": "gemm_bf16_kernel<256,128,..,EpiStage1,64,512>",
+             "wide": "gemm_bf16_kernel<128,128,..,EpiStage1,128>" if dom == "bf16" else None,
+             }.get(choice) or ("gemm_bf16_kernel<128,128,..,EpiStage1,64,256,DB=0,MINW=3>" if dom == "bf16"
+                               else "gemm_bf16_kernel<128,128,..,EpiStage1,64>")
     kname = kname.replace("<", "<" + ("split," if dom == "bf16x3" else "plain,"), 1)
-    if dom == "bf16" and kname.endswith(",64>") and "512" not in kname:
-        kname = kname[:-4] + ",128>"   # plain BF16 takes 128-wide K steps
     traffic = (HD_STAGE1_TRAFFIC[dom]["read"] + HD_STAGE1_TRAFFIC[dom]["write"]
                if dom in HD_STAGE1_TRAFFIC and "512" not in kname and "256,256" not in kname and "pipe" not in kname
                else None)

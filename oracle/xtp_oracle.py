@@ -8,6 +8,19 @@ Every function cites the reference file:line whose loop it restates. Layout foll
 the reference: an Mmn tensor is a list of mtotal slices, each (ntotal x P); here it is
 one numpy array M[m, n, P] (m = slice index, n = row, P = aux column), so that
 M[m] is exactly the reference's Mmn_[m] (xtp/include/votca/xtp/threecenter.h:125).
+
+What pins each function (tests/test_oracle_golden.py): fill_mmn / multiply_right by the
+threecenter_gwbse fixtures; rpa_epsilon by the rpa fixtures and brute force; rpa_update_energies
+by its closed form; rpa_h2p by the rpa_h2p known answers and the RPA-matrix spectrum; ppm_construct
+and sigma_ppm_* by the ppm known answers and sigma_ppm fixtures; sigma_x by x_ref; qp_* by gw/ref
+and the single-pole closed form; bse_dense / BSEOperatorOracle by the bse_operator fixtures and
+the bse energy / coefficient fixtures; DavidsonOracle.solve / solve_ham by dense eigensolvers and
+the bse fixtures; screening_at / perturbative_dynamical_screening by the bse dynamic fixtures;
+SigmaCDAOracle by the sigma_cda fixtures; pseudo_invsqrt_gwbse through the threecenter fixtures;
+the synthetic generators by their distribution tests. Parity unpinned: the Davidson's iteration
+counts and intermediate iterates (only converged eigenpairs are pinned), and the phase convention
+of a complex harmonic-Ritz eigenvector before its real part is taken (LAPACK's here, Eigen's QZ
+back-substitution in the reference).
 """
 from __future__ import annotations
 

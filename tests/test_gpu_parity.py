@@ -301,24 +301,30 @@ def test_bse_operator_random_mixed(ctx, c, prec, tol):
 @pytest.mark.parametrize("shape", [(11, 90, 130, 7), (40, 200, 300, 8)])
 @pytest.mark.parametrize("c", [(0, 0, 1, 0), (0, 0, 0, 1)])
 def test_bf16_kernels_agree(ctx, c, shape, monkeypatch):
-    """The three split-BF16 GEMM variants (register-staged with 32- and 64-wide K steps, and the
-    direct-to-LDS pipeline) on the Hd / Hd2 stages, against each other and FP64; the larger shape
-    takes the pipeline's 256x128 tile, the smaller its 128x128 one, both with ragged edges (row
-    and K extents that are not tile multiples)."""
+    """The BF16 GEMM variants (register-staged with 32-, 64- and 128-wide K steps, one or two LDS
+    stages, register caps for 3 or 4 waves per SIMD, and the direct-to-LDS pipeline) on the Hd /
+    Hd2 stages, split and plain, against each other and FP64; the larger shape takes the
+    pipeline's 256x128 tile, the smaller its 128x128 one, both with ragged edges (row and K
+    extents that are not tile multiples)."""
     vt, ct, P, k = shape
     M, eps, Hqp, homo, rpamin, vmin, cmax = random_bse_case(14, vt=vt, ct=ct, P=P)
     t = dev_mmn(ctx, M)
-    op = px.BSE_OPERATOR(*c, eps, t, Hqp, precisions=("fp64", "bf16x3")).configure(
+    op = px.BSE_OPERATOR(*c, eps, t, Hqp, precisions=("fp64", "bf16", "bf16x3")).configure(
         px.BSEOperator_Options(homo, rpamin, 0, vmin, cmax))
     Xv = np.random.default_rng(15).normal(size=(op.size, k))
     ref = op.matmul(Xv, "fp64")
-    ys = {}
-    for kern in ("staged32", "staged64", "pipe", "big", "tall"):
+    ys, yp = {}, {}
+    for kern in ("staged32", "staged64", "wide", "pipe", "big", "tall", "db64", "db128", "k64occ3", "k32occ3"):
         monkeypatch.setenv("XTPGPU_BF16_KERNEL", kern)
         ys[kern] = op.matmul(Xv, "bf16x3")
         assert relerr(ys[kern], ref) < 2e-5, kern
+        yp[kern] = op.matmul(Xv, "bf16")
+        assert relerr(yp[kern], ref) < 3e-2, kern
     assert relerr(ys["pipe"], ys["staged32"]) < 2e-5
     assert relerr(ys["staged64"], ys["staged32"]) < 2e-5
+    # the plain forms differ only in FP32 summation order
+    for kern in yp:
+        assert relerr(yp[kern], yp["staged32"]) < 1e-4, kern
 
 
 @pytest.mark.parametrize("vt,ct,P", [(1, 1, 1), (1, 17, 3), (9, 1, 5), (3, 8, 1)])

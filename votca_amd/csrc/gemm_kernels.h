@@ -304,7 +304,8 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmDims d, Opnd<double> 
 // on load and re-split into hi/lo, so the weight costs no extra precision.
 // Block tile BM x BN x BKT, NT threads = (NT/128) x 2 waves.
 // VEC: operands are K-contiguous with ld, base and K bounds multiples of 8 (16-byte loads).
-template <int BM, int BN, bool AK, bool BK_, bool SPLIT, bool HAS_S, bool VEC, class Epi, int BKT = 32, int NT = 256>
+template <int BM, int BN, bool AK, bool BK_, bool SPLIT, bool HAS_S, bool VEC, class Epi, int BKT = 32, int NT = 256,
+          bool DB = false>
 __device__ __forceinline__ void gemm_bf16_tile(const GemmDims& d, const Opnd<unsigned short>& A,
                                                const Opnd<unsigned short>& B, const float* __restrict__ s,
                                                long s_bs, const Epi& epi, int tm, int tn, int b, int ks) {
@@ -322,8 +323,12 @@ __device__ __forceinline__ void gemm_bf16_tile(const GemmDims& d, const Opnd<uns
   constexpr int NS = SPLIT ? 2 : 1;
   constexpr int LA = BM * BK / NT;
   constexpr int LB = BN * BK / NT;
-  __shared__ __attribute__((aligned(16))) unsigned short As[NS][BM * PK];
-  __shared__ __attribute__((aligned(16))) unsigned short Bs[NS][BN * PK];
+  // DB: two LDS stages; the next panel is stored into the idle stage after the MFMAs of the
+  // current one, so a K step needs one barrier instead of two
+  constexpr int NB = DB ? 2 : 1;
+  __shared__ __attribute__((aligned(16))) unsigned short As_[NB][NS][BM * PK];
+  __shared__ __attribute__((aligned(16))) unsigned short Bs_[NB][NS][BN * PK];
+  int stage = 0;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -398,13 +403,13 @@ __device__ __forceinline__ void gemm_bf16_tile(const GemmDims& d, const Opnd<uns
       for (int i = 0; i < VA; ++i) {
         int vid = tid + i * NT;
         int mm = vid / (BK / 8), kq = 8 * (vid % (BK / 8));
-        *reinterpret_cast<us8v*>(&As[q][mm * PK + kq]) = va[q][i];
+        *reinterpret_cast<us8v*>(&As_[stage][q][mm * PK + kq]) = va[q][i];
       }
 #pragma unroll
       for (int i = 0; i < VB; ++i) {
         int vid = tid + i * NT;
         int nn = vid / (BK / 8), kq = 8 * (vid % (BK / 8));
-        *reinterpret_cast<us8v*>(&Bs[q][nn * PK + kq]) = vb[q][i];
+        *reinterpret_cast<us8v*>(&Bs_[stage][q][nn * PK + kq]) = vb[q][i];
       }
     }
   };
@@ -456,38 +461,32 @@ __device__ __forceinline__ void gemm_bf16_tile(const GemmDims& d, const Opnd<uns
         int idx = tid + i * NT;
         int mm, kk;
         if (AK) { mm = idx / BK; kk = idx % BK; } else { mm = idx % BM; kk = idx / BM; }
-        As[q][mm * PK + kk] = ra[q][i];
+        As_[stage][q][mm * PK + kk] = ra[q][i];
       }
 #pragma unroll
       for (int i = 0; i < LB; ++i) {
         int idx = tid + i * NT;
         int nn, kk;
         if (BK_) { nn = idx / BK; kk = idx % BK; } else { nn = idx % BN; kk = idx / BN; }
-        Bs[q][nn * PK + kk] = rb[q][i];
+        Bs_[stage][q][nn * PK + kk] = rb[q][i];
       }
     }
   };
 
-  if (kbeg < kend) {
-    load(kbeg);
-    for (int k0 = kbeg; k0 < kend; k0 += BK) {
-      __syncthreads();
-      store();
-      __syncthreads();
-      if (k0 + BK < kend) load(k0 + BK);
+  auto compute = [&](int st_) {
 #pragma unroll
-      for (int kk = 0; kk < BK / 32; ++kk) {
+    for (int kk = 0; kk < BK / 32; ++kk) {
       const int kq = kk * 32 + 8 * (lane >> 4);
       bf8v ah[TM], bh[TN], al[TM], bl[TN];
 #pragma unroll
       for (int i = 0; i < TM; ++i) {
-        ah[i] = *reinterpret_cast<const bf8v*>(&As[0][(wm0 + i * 16 + (lane & 15)) * PK + kq]);
-        if (SPLIT) al[i] = *reinterpret_cast<const bf8v*>(&As[NS - 1][(wm0 + i * 16 + (lane & 15)) * PK + kq]);
+        ah[i] = *reinterpret_cast<const bf8v*>(&As_[st_][0][(wm0 + i * 16 + (lane & 15)) * PK + kq]);
+        if (SPLIT) al[i] = *reinterpret_cast<const bf8v*>(&As_[st_][NS - 1][(wm0 + i * 16 + (lane & 15)) * PK + kq]);
       }
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
-        bh[j] = *reinterpret_cast<const bf8v*>(&Bs[0][(wn0 + j * 16 + (lane & 15)) * PK + kq]);
-        if (SPLIT) bl[j] = *reinterpret_cast<const bf8v*>(&Bs[NS - 1][(wn0 + j * 16 + (lane & 15)) * PK + kq]);
+        bh[j] = *reinterpret_cast<const bf8v*>(&Bs_[st_][0][(wn0 + j * 16 + (lane & 15)) * PK + kq]);
+        if (SPLIT) bl[j] = *reinterpret_cast<const bf8v*>(&Bs_[st_][NS - 1][(wn0 + j * 16 + (lane & 15)) * PK + kq]);
       }
 #pragma unroll
       for (int i = 0; i < TM; ++i)
@@ -499,6 +498,34 @@ __device__ __forceinline__ void gemm_bf16_tile(const GemmDims& d, const Opnd<uns
             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
           }
         }
+    }
+  };
+
+  if (kbeg < kend) {
+    if constexpr (DB) {
+      // stage s holds panel k0 while the registers carry panel k0 + BK; every wave has left
+      // stage s ^ 1 (the barrier closing the previous step) before it is overwritten
+      load(kbeg);
+      store();
+      __syncthreads();
+      for (int k0 = kbeg; k0 < kend; k0 += BK) {
+        const bool more = k0 + BK < kend;
+        if (more) load(k0 + BK);
+        compute(stage);
+        if (more) {
+          stage ^= 1;
+          store();
+        }
+        __syncthreads();
+      }
+    } else {
+      load(kbeg);
+      for (int k0 = kbeg; k0 < kend; k0 += BK) {
+        __syncthreads();
+        store();
+        __syncthreads();
+        if (k0 + BK < kend) load(k0 + BK);
+        compute(0);
       }
     }
   }
@@ -508,12 +535,14 @@ __device__ __forceinline__ void gemm_bf16_tile(const GemmDims& d, const Opnd<uns
   __syncthreads();  // LDS is reused by the next tile of a persistent caller
 }
 
-template <int BM, int BN, bool AK, bool BK_, bool SPLIT, bool HAS_S, bool VEC, class Epi, int BKT = 32, int NT = 256>
-__global__ __launch_bounds__(NT) void gemm_bf16_kernel(GemmDims d, Opnd<unsigned short> A, Opnd<unsigned short> B,
-                                                        const float* __restrict__ s, long s_bs, Epi epi) {
+template <int BM, int BN, bool AK, bool BK_, bool SPLIT, bool HAS_S, bool VEC, class Epi, int BKT = 32, int NT = 256,
+          bool DB = false, int MINW = 1>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW))) void gemm_bf16_kernel(
+    GemmDims d, Opnd<unsigned short> A, Opnd<unsigned short> B, const float* __restrict__ s, long s_bs, Epi epi) {
   int tm, tn;
   tile_of_block(d, tm, tn);
-  gemm_bf16_tile<BM, BN, AK, BK_, SPLIT, HAS_S, VEC, Epi, BKT, NT>(d, A, B, s, s_bs, epi, tm, tn, blockIdx.z, blockIdx.y);
+  gemm_bf16_tile<BM, BN, AK, BK_, SPLIT, HAS_S, VEC, Epi, BKT, NT, DB>(d, A, B, s, s_bs, epi, tm, tn, blockIdx.z,
+                                                                       blockIdx.y);
 }
 
 // ------------------------------------------------------------------ host launch helpers
@@ -602,6 +631,57 @@ inline hipError_t launch_gemm_bf16_tall(hipStream_t st, const GemmDims& d, Opnd<
   } else {
     if (s) hipLaunchKernelGGL((gemm_bf16_kernel<256, 128, AK, BK_, false, true, true, Epi, 64, 512>), grid, dim3(512), 0, st, d, A, B, s, s_bs, epi);
     else hipLaunchKernelGGL((gemm_bf16_kernel<256, 128, AK, BK_, false, false, true, Epi, 64, 512>), grid, dim3(512), 0, st, d, A, B, s, s_bs, epi);
+  }
+  return hipGetLastError();
+}
+
+// 256 x 128 block tile with 256 threads (wave tile 128 x 64: 32 accumulator tiles per wave),
+// K step BKT, vector loads: 1.5x the MFMAs per LDS fragment read of the 64 x 64 wave tile.
+template <int BKT, bool AK, bool BK_, class Epi>
+inline hipError_t launch_gemm_bf16_wave128(hipStream_t st, const GemmDims& d, Opnd<unsigned short> A,
+                                           Opnd<unsigned short> B, const float* s, long s_bs, bool split, Epi epi) {
+  static_assert(AK && BK_, "K-contiguous operands only");
+  dim3 grid(gemm_tiles(d), d.ksplit, d.batch);
+  if (split) {
+    if (s) hipLaunchKernelGGL((gemm_bf16_kernel<256, 128, AK, BK_, true, true, true, Epi, 64, 256>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+    else hipLaunchKernelGGL((gemm_bf16_kernel<256, 128, AK, BK_, true, false, true, Epi, 64, 256>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+  } else {
+    if (s) hipLaunchKernelGGL((gemm_bf16_kernel<256, 128, AK, BK_, false, true, true, Epi, BKT, 256>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+    else hipLaunchKernelGGL((gemm_bf16_kernel<256, 128, AK, BK_, false, false, true, Epi, BKT, 256>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+  }
+  return hipGetLastError();
+}
+
+// 128 x 128 block tile, 256 threads, two LDS stages (one barrier per K step), K step BKT
+// (64 for split operands), vector loads.
+template <int BKT, bool AK, bool BK_, class Epi>
+inline hipError_t launch_gemm_bf16_db(hipStream_t st, const GemmDims& d, Opnd<unsigned short> A,
+                                      Opnd<unsigned short> B, const float* s, long s_bs, bool split, Epi epi) {
+  static_assert(AK && BK_, "K-contiguous operands only");
+  dim3 grid(gemm_tiles(d), d.ksplit, d.batch);
+  if (split) {
+    if (s) hipLaunchKernelGGL((gemm_bf16_kernel<128, 128, AK, BK_, true, true, true, Epi, 64, 256, true>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+    else hipLaunchKernelGGL((gemm_bf16_kernel<128, 128, AK, BK_, true, false, true, Epi, 64, 256, true>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+  } else {
+    if (s) hipLaunchKernelGGL((gemm_bf16_kernel<128, 128, AK, BK_, false, true, true, Epi, BKT, 256, true>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+    else hipLaunchKernelGGL((gemm_bf16_kernel<128, 128, AK, BK_, false, false, true, Epi, BKT, 256, true>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+  }
+  return hipGetLastError();
+}
+
+// 128 x 128 block tile, 256 threads, one LDS stage, K step BKT for both forms, compiled for at
+// least MINW waves per SIMD (registers capped accordingly).
+template <int MINW, int BKT, bool AK, bool BK_, class Epi>
+inline hipError_t launch_gemm_bf16_occ(hipStream_t st, const GemmDims& d, Opnd<unsigned short> A,
+                                       Opnd<unsigned short> B, const float* s, long s_bs, bool split, Epi epi) {
+  static_assert(AK && BK_, "K-contiguous operands only");
+  dim3 grid(gemm_tiles(d), d.ksplit, d.batch);
+  if (split) {
+    if (s) hipLaunchKernelGGL((gemm_bf16_kernel<128, 128, AK, BK_, true, true, true, Epi, BKT, 256, false, MINW>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+    else hipLaunchKernelGGL((gemm_bf16_kernel<128, 128, AK, BK_, true, false, true, Epi, BKT, 256, false, MINW>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+  } else {
+    if (s) hipLaunchKernelGGL((gemm_bf16_kernel<128, 128, AK, BK_, false, true, true, Epi, BKT, 256, false, MINW>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+    else hipLaunchKernelGGL((gemm_bf16_kernel<128, 128, AK, BK_, false, false, true, Epi, BKT, 256, false, MINW>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
   }
   return hipGetLastError();
 }

@@ -186,15 +186,28 @@ void gemm64(hipStream_t st, int M, int N, int K, int batch, int ksplit, Opnd<dou
   XG_HIP((launch_gemm_f64<64, 64, AK, BK_>(st, d, A, B, s, s_bs, epi)));
 }
 
-// BF16 kernel choice: 0 = register-staged BK 32, 1 = register-staged BK 64 (K-contiguous
-// operands), 2 = direct-to-LDS pipeline (split, K-contiguous, unscaled). XTPGPU_BF16_KERNEL
-// = staged32 | staged64 | pipe overrides the default for A/B timing.
+// BF16 kernel choice (XTPGPU_BF16_KERNEL, for A/B timing; anything else is the default, 1):
+// 1 = K-contiguous operands: split forms 128x128 with BK 64 (two workgroups per CU), plain BF16
+// 128x128 with BK 64 capped for three waves per SIMD; staged32 = BK 32; wide = split BK 64 /
+// plain BK 128 at two workgroups per CU; pipe = direct-to-LDS pipeline; big = 256x256; tall =
+// 256x128 with 512 threads; wave128 / wave128k128 = 256x128 with 256 threads; db64 / db128 = two
+// LDS stages; k64occ{2,3,4} / k32occ{3,4} = register caps for 2-4 waves per SIMD.
 inline int bf16_kernel_choice() {
   const char* e = std::getenv("XTPGPU_BF16_KERNEL");
   if (e && std::strcmp(e, "staged32") == 0) return 0;
   if (e && std::strcmp(e, "pipe") == 0) return 2;
   if (e && std::strcmp(e, "big") == 0) return 3;
   if (e && std::strcmp(e, "tall") == 0) return 4;
+  if (e && std::strcmp(e, "wave128") == 0) return 5;
+  if (e && std::strcmp(e, "wave128k128") == 0) return 6;
+  if (e && std::strcmp(e, "db64") == 0) return 7;
+  if (e && std::strcmp(e, "db128") == 0) return 8;
+  if (e && std::strcmp(e, "k64occ2") == 0) return 9;
+  if (e && std::strcmp(e, "k64occ3") == 0) return 10;
+  if (e && std::strcmp(e, "k64occ4") == 0) return 11;
+  if (e && std::strcmp(e, "k32occ3") == 0) return 12;
+  if (e && std::strcmp(e, "k32occ4") == 0) return 13;
+  if (e && std::strcmp(e, "wide") == 0) return 14;
   return 1;
 }
 
@@ -209,13 +222,52 @@ void gemm16(hipStream_t st, int M, int N, int K, int batch, int ksplit, Opnd<uns
       XG_HIP((launch_gemm_bf16_big<AK, BK_>(st, d, A, B, s, s_bs, split, epi)));
       return;
     }
-    if (choice == 4 && bf16_vec(A, B, K, s_bs) && (long)M * N >= 256L * 128 * 512) {
-      GemmDims d = make_dims<256, 128>(M, N, std::max(K, 0), batch, ksplit, 64, lower);
-      if (!lower) d.group = tile_group(d);
+    // the lower-triangle tile walk assumes square tiles: 256 x 128 only for full outputs
+    if (choice == 4 && !lower && bf16_vec(A, B, K, s_bs) && (long)M * N >= 256L * 128 * 512) {
+      GemmDims d = make_dims<256, 128>(M, N, std::max(K, 0), batch, ksplit, 64, false);
+      d.group = tile_group(d);
       XG_HIP((launch_gemm_bf16_tall<AK, BK_>(st, d, A, B, s, s_bs, split, epi)));
       return;
     }
-    if ((choice == 1 || choice == 3 || choice == 4) && bf16_vec(A, B, K, s_bs)) {
+    if ((choice == 5 || choice == 6) && !lower && bf16_vec(A, B, K, s_bs) && (long)M * N >= 256L * 128 * 512) {
+      const int bk = split ? 64 : (choice == 6 ? 128 : 64);
+      GemmDims d = make_dims<256, 128>(M, N, std::max(K, 0), batch, ksplit, bk, false);
+      d.group = tile_group(d);
+      if (bk == 128)
+        XG_HIP((launch_gemm_bf16_wave128<128, AK, BK_>(st, d, A, B, s, s_bs, split, epi)));
+      else
+        XG_HIP((launch_gemm_bf16_wave128<64, AK, BK_>(st, d, A, B, s, s_bs, split, epi)));
+      return;
+    }
+    if ((choice == 7 || choice == 8) && bf16_vec(A, B, K, s_bs)) {
+      const int bk = split ? 64 : (choice == 8 ? 128 : 64);
+      GemmDims d = make_dims<128, 128>(M, N, std::max(K, 0), batch, ksplit, bk, lower);
+      if (!lower) d.group = tile_group(d);
+      if (bk == 128)
+        XG_HIP((launch_gemm_bf16_db<128, AK, BK_>(st, d, A, B, s, s_bs, split, epi)));
+      else
+        XG_HIP((launch_gemm_bf16_db<64, AK, BK_>(st, d, A, B, s, s_bs, split, epi)));
+      return;
+    }
+    // default for plain BF16: 64-wide K steps with the registers capped for three waves per SIMD
+    // (148 VGPRs, 37 KB LDS: three workgroups per CU) -- latency hiding by occupancy
+    if (choice == 1 && !split && bf16_vec(A, B, K, s_bs)) {
+      GemmDims d = make_dims<128, 128>(M, N, std::max(K, 0), batch, ksplit, 64, lower);
+      if (!lower) d.group = tile_group(d);
+      XG_HIP((launch_gemm_bf16_occ<3, 64, AK, BK_>(st, d, A, B, s, s_bs, split, epi)));
+      return;
+    }
+    if (choice >= 9 && choice <= 13 && bf16_vec(A, B, K, s_bs)) {
+      GemmDims d = make_dims<128, 128>(M, N, std::max(K, 0), batch, ksplit, choice >= 12 ? 32 : 64, lower);
+      if (!lower) d.group = tile_group(d);
+      if (choice == 9) XG_HIP((launch_gemm_bf16_occ<2, 64, AK, BK_>(st, d, A, B, s, s_bs, split, epi)));
+      else if (choice == 10) XG_HIP((launch_gemm_bf16_occ<3, 64, AK, BK_>(st, d, A, B, s, s_bs, split, epi)));
+      else if (choice == 11) XG_HIP((launch_gemm_bf16_occ<4, 64, AK, BK_>(st, d, A, B, s, s_bs, split, epi)));
+      else if (choice == 12) XG_HIP((launch_gemm_bf16_occ<3, 32, AK, BK_>(st, d, A, B, s, s_bs, split, epi)));
+      else XG_HIP((launch_gemm_bf16_occ<4, 32, AK, BK_>(st, d, A, B, s, s_bs, split, epi)));
+      return;
+    }
+    if ((choice == 1 || choice >= 3) && bf16_vec(A, B, K, s_bs)) {
       GemmDims d = make_dims<128, 128>(M, N, std::max(K, 0), batch, ksplit, bf16_wide_kstep(split), lower);
       if (!lower) d.group = tile_group(d);
       XG_HIP((launch_gemm_bf16<128, 128, AK, BK_>(st, d, A, B, s, s_bs, split, epi, true)));
