@@ -305,7 +305,7 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmDims d, Opnd<double> 
 // Block tile BM x BN x BKT, NT threads = (NT/128) x 2 waves.
 // VEC: operands are K-contiguous with ld, base and K bounds multiples of 8 (16-byte loads).
 template <int BM, int BN, bool AK, bool BK_, bool SPLIT, bool HAS_S, bool VEC, class Epi, int BKT = 32, int NT = 256,
-          bool DB = false>
+          bool DB = false, int PAD = 8>
 __device__ __forceinline__ void gemm_bf16_tile(const GemmDims& d, const Opnd<unsigned short>& A,
                                                const Opnd<unsigned short>& B, const float* __restrict__ s,
                                                long s_bs, const Epi& epi, int tm, int tn, int b, int ks) {
@@ -316,7 +316,8 @@ __device__ __forceinline__ void gemm_bf16_tile(const GemmDims& d, const Opnd<uns
   // padded rows are conflict-free as well.
   constexpr int BK = BKT;
   static_assert(BK == 32 || ((BK == 64 || BK == 128) && VEC), "BK 64 / 128 need vector loads");
-  constexpr int PK = BK + 8;  // padded LDS row (bf16 elements), rows stay 16-byte aligned
+  static_assert(PAD % 8 == 0, "LDS rows must stay 16-byte aligned");
+  constexpr int PK = BK + PAD;  // padded LDS row (bf16 elements)
   static_assert(NT == 256 || NT == 512, "256 or 512 threads");
   constexpr int WM = BM / (NT / 128), WN = BN / 2;
   constexpr int TM = WM / 16, TN = WN / 16;
@@ -536,12 +537,12 @@ __device__ __forceinline__ void gemm_bf16_tile(const GemmDims& d, const Opnd<uns
 }
 
 template <int BM, int BN, bool AK, bool BK_, bool SPLIT, bool HAS_S, bool VEC, class Epi, int BKT = 32, int NT = 256,
-          bool DB = false, int MINW = 1>
+          bool DB = false, int MINW = 1, int PAD = 8>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW))) void gemm_bf16_kernel(
     GemmDims d, Opnd<unsigned short> A, Opnd<unsigned short> B, const float* __restrict__ s, long s_bs, Epi epi) {
   int tm, tn;
   tile_of_block(d, tm, tn);
-  gemm_bf16_tile<BM, BN, AK, BK_, SPLIT, HAS_S, VEC, Epi, BKT, NT, DB>(d, A, B, s, s_bs, epi, tm, tn, blockIdx.z,
+  gemm_bf16_tile<BM, BN, AK, BK_, SPLIT, HAS_S, VEC, Epi, BKT, NT, DB, PAD>(d, A, B, s, s_bs, epi, tm, tn, blockIdx.z,
                                                                        blockIdx.y);
 }
 
@@ -671,17 +672,17 @@ inline hipError_t launch_gemm_bf16_db(hipStream_t st, const GemmDims& d, Opnd<un
 
 // 128 x 128 block tile, 256 threads, one LDS stage, K step BKT for both forms, compiled for at
 // least MINW waves per SIMD (registers capped accordingly).
-template <int MINW, int BKT, bool AK, bool BK_, class Epi>
+template <int MINW, int BKT, bool AK, bool BK_, class Epi, int PAD = 8>
 inline hipError_t launch_gemm_bf16_occ(hipStream_t st, const GemmDims& d, Opnd<unsigned short> A,
                                        Opnd<unsigned short> B, const float* s, long s_bs, bool split, Epi epi) {
   static_assert(AK && BK_, "K-contiguous operands only");
   dim3 grid(gemm_tiles(d), d.ksplit, d.batch);
   if (split) {
-    if (s) hipLaunchKernelGGL((gemm_bf16_kernel<128, 128, AK, BK_, true, true, true, Epi, BKT, 256, false, MINW>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
-    else hipLaunchKernelGGL((gemm_bf16_kernel<128, 128, AK, BK_, true, false, true, Epi, BKT, 256, false, MINW>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+    if (s) hipLaunchKernelGGL((gemm_bf16_kernel<128, 128, AK, BK_, true, true, true, Epi, BKT, 256, false, MINW, PAD>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+    else hipLaunchKernelGGL((gemm_bf16_kernel<128, 128, AK, BK_, true, false, true, Epi, BKT, 256, false, MINW, PAD>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
   } else {
-    if (s) hipLaunchKernelGGL((gemm_bf16_kernel<128, 128, AK, BK_, false, true, true, Epi, BKT, 256, false, MINW>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
-    else hipLaunchKernelGGL((gemm_bf16_kernel<128, 128, AK, BK_, false, false, true, Epi, BKT, 256, false, MINW>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+    if (s) hipLaunchKernelGGL((gemm_bf16_kernel<128, 128, AK, BK_, false, true, true, Epi, BKT, 256, false, MINW, PAD>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+    else hipLaunchKernelGGL((gemm_bf16_kernel<128, 128, AK, BK_, false, false, true, Epi, BKT, 256, false, MINW, PAD>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
   }
   return hipGetLastError();
 }
