@@ -243,6 +243,52 @@ def run_matvec(args):
     print(json.dumps(out))
 
 
+def run_large_eps(args):
+    """eps(i w) of the large single segment (default n_occ=250, n_virt=2250, n_aux=6000): the
+    transition block M^P_{vc} (27 GB FP64) resident in HBM, eps at the 20 Gauss-Legendre points
+    of the CDA default scheme (gwbse.xml:18-19), FP64 and the mixed form; TFLOP/s from the
+    library's HIP-event timer and the relative error of the mixed eps on three frequencies."""
+    from votca_amd import pyxtp as px
+    from votca_amd import quadrature
+    o, v, P = args.nocc, args.nvirt, args.naux
+    n = o + v
+    ctx = px.Context(0)
+    log(f"[large-eps] o={o} v={v} P={P}: M^P_(v,c) window {o*n*P*8/1e9:.1f} GB FP64")
+    Mmn = px.TCMatrix_gwbse(ctx).Initialize(P, 0, o - 1, 0, n - 1)
+    Mmn.fill_synthetic(args.seed, args.sigma0, 3.0)
+    e = synthetic_energies(args.seed, o, v)
+    nf = args.nfreq
+    freqs = np.ascontiguousarray(quadrature.create("legendre", nf).points())
+    F = float(nf) * o * v * P * (P + 1)
+    out_dev = px.DeviceBuffer(ctx, P * P * nf * 8)
+    ctx.set_profiling(True)
+    sample = sorted({0, nf // 2, nf - 1})
+    res, ref = {}, {}
+    for prec in ("fp64", args.mixed):
+        ctx.reset_timers()
+        t0 = time.perf_counter()
+        px._check(px.lib().xtpgpu_rpa_epsilon_device(ctx.h, Mmn.h, px._dptr(e), o - 1, 0, n - 1, px.EPS_IMAG,
+                                                     px._dptr(freqs), None, nf, 1e-4, px._PREC[prec], out_dev.ptr))
+        ctx.synchronize()
+        wall = time.perf_counter() - t0
+        tm = ctx.timer("rpa_eps_gemm")
+        res[prec] = {"wall_s": wall, "kernel_s": tm["ms"] / 1e3, "tflops": F / (tm["ms"] / 1e3) / 1e12}
+        errs = []
+        for f in sample:
+            slab = out_dev.download((P, P), offset=f * P * P * 8)
+            if prec == "fp64":
+                ref[f] = slab
+            else:
+                errs.append(float(np.linalg.norm(slab - ref[f]) / np.linalg.norm(ref[f])))
+        if errs:
+            res[prec]["max_rel_frobenius_err"] = max(errs)
+        log(f"[large-eps] {prec}: kernel {res[prec]['kernel_s']:.2f} s ({res[prec]['tflops']:.1f} TFLOP/s), "
+            f"wall {wall:.2f} s")
+    res["speedup"] = res["fp64"]["kernel_s"] / res[args.mixed]["kernel_s"]
+    print(json.dumps({"workload": f"large single segment eps(i w), n_occ={o} n_virt={v} n_aux={P}, {nf} frequencies",
+                      "algorithmic_flops": F, "sampled_frequencies": sample, **res}))
+
+
 def run_adjoint(args):
     """Full (non-TDA) singlet BSE for `--roots` roots, then the Hellmann-Feynman derivatives of the
     lowest `--adj-roots` with respect to the QP energies (all levels) and the transition block
@@ -392,7 +438,8 @@ def main():
     ap.add_argument("--switch-tol", type=float, default=1e-5)
     ap.add_argument("--refine-max", type=int, default=8, help="FP64 refinement rounds of the mixed solve")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="single", choices=["single", "segments", "matvec", "adjoint"])
+    ap.add_argument("--workload", default="single", choices=["single", "segments", "matvec", "adjoint", "large-eps"])
+    ap.add_argument("--nfreq", type=int, default=20, help="large-eps workload: Gauss-Legendre points")
     ap.add_argument("--adj-roots", type=int, default=5)
     ap.add_argument("--fd-levels", type=int, default=3)
     ap.add_argument("--fd-mmn", type=int, default=3)
@@ -420,6 +467,9 @@ def main():
         return
     if args.workload == "adjoint":
         run_adjoint(args)
+        return
+    if args.workload == "large-eps":
+        run_large_eps(args)
         return
     from votca_amd import pyxtp as px
     from votca_amd import quadrature

@@ -639,36 +639,39 @@ extern "C" int xtpgpu_mmn_set_block(xtpgpu_mmn mmn, int64_t vmin, int64_t vtotal
 namespace {
 
 struct RpaPacked {
-  DevBuf A64;          // [P][K] FP64
-  DevBuf Ah, Al;       // [P][K] bf16 hi / lo
-  long K = 0;
+  DevBuf A64;          // [P][Kp] FP64
+  DevBuf Ah, Al;       // [P][Kp] bf16 hi / lo
+  long K = 0;          // transitions n_occ * n_unocc
+  long Kp = 0;         // row length: K rounded up to 8 (16-byte rows for the vector loads), zero pads
 };
 
 void rpa_pack(xtpgpu_mmn mmn, long n_occ, long n_unocc, bool need64, bool need16, RpaPacked& pk) {
   hipStream_t st = mmn->ctx->stream;
   const long P = mmn->aux, K = n_occ * n_unocc, nt = mmn->ntotal, ps = (long)mmn->pstride();
+  const long Kp = (K + 7) / 8 * 8;
   const long r0 = nt - n_unocc;
   pk.K = K;
+  pk.Kp = Kp;
   const double* D = mmn->d();
   if (need64) {
-    pk.A64.alloc((size_t)P * K * sizeof(double));
+    pk.A64.alloc((size_t)P * Kp * sizeof(double));
     double* A = pk.A64.as<double>();
-    launch_map(st, (size_t)P * K, [=] __device__(size_t i) {
-      long p = (long)(i / K), k = (long)(i % K);
+    launch_map(st, (size_t)P * Kp, [=] __device__(size_t i) {
+      long p = (long)(i / Kp), k = (long)(i % Kp);
       long v = k / n_unocc, c = k % n_unocc;
-      A[i] = D[p * ps + v * nt + r0 + c];
+      A[i] = k < K ? D[p * ps + v * nt + r0 + c] : 0.0;
     });
   }
   if (need16) {
-    pk.Ah.alloc((size_t)P * K * 2);
-    pk.Al.alloc((size_t)P * K * 2);
+    pk.Ah.alloc((size_t)P * Kp * 2);
+    pk.Al.alloc((size_t)P * Kp * 2);
     unsigned short* H = pk.Ah.as<unsigned short>();
     unsigned short* L = pk.Al.as<unsigned short>();
-    launch_map(st, (size_t)P * K, [=] __device__(size_t i) {
-      long p = (long)(i / K), k = (long)(i % K);
+    launch_map(st, (size_t)P * Kp, [=] __device__(size_t i) {
+      long p = (long)(i / Kp), k = (long)(i % Kp);
       long v = k / n_unocc, c = k % n_unocc;
-      unsigned short h, l;
-      split_bf16(D[p * ps + v * nt + r0 + c], h, l);
+      unsigned short h = 0, l = 0;
+      if (k < K) split_bf16(D[p * ps + v * nt + r0 + c], h, l);
       H[i] = h;
       L[i] = l;
     });
@@ -707,14 +710,19 @@ void rpa_epsilon_impl(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* energies, in
     w[n_omega + i] = omega_im ? omega_im[i] : 0.0;
   }
   XG_HIP(hipMemcpyAsync(dw.p, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice, st));
-  DevBuf dd((size_t)n_omega * K * (use16 ? sizeof(float) : sizeof(double)));
+  const long Kp = pk.Kp;
+  DevBuf dd((size_t)n_omega * Kp * (use16 ? sizeof(float) : sizeof(double)));
   const double* E = de.as<double>();
   const double* W = dw.as<double>();
   double* d64 = use16 ? nullptr : dd.as<double>();
   float* d32 = use16 ? dd.as<float>() : nullptr;
   const int nw = n_omega;
-  launch_map(st, (size_t)n_omega * K, [=] __device__(size_t i) {
-    long f = (long)(i / K), k = (long)(i % K);
+  launch_map(st, (size_t)n_omega * Kp, [=] __device__(size_t i) {
+    long f = (long)(i / Kp), k = (long)(i % Kp);
+    if (k >= K) {  // pad transitions carry zero weight
+      if (d64) d64[i] = 0.0; else d32[i] = 0.0f;
+      return;
+    }
     long v = k / n_unocc, c = k % n_unocc;
     double D = E[n_occ + c] - E[v];
     double wr = W[f], wi = W[nw + f];
@@ -737,12 +745,12 @@ void rpa_epsilon_impl(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* energies, in
   ProfScope prof(ctx, "rpa_eps_gemm", (double)n_omega * K * P * (P + 1),
                  (double)P * K * (use16 ? (precision == XTPGPU_BF16X3 ? 4 : 2) : 8) + (double)n_omega * P * P * 8);
   if (!use16) {
-    Opnd<double> A{pk.A64.as<double>(), nullptr, K, 0};
-    gemm64<true, true>(st, (int)P, (int)P, (int)K, n_omega, 1, A, A, d64, K, true, epi);
+    Opnd<double> A{pk.A64.as<double>(), nullptr, Kp, 0};
+    gemm64<true, true>(st, (int)P, (int)P, (int)Kp, n_omega, 1, A, A, d64, Kp, true, epi);
   } else {
     const bool split = precision == XTPGPU_BF16X3;
-    Opnd<unsigned short> A{pk.Ah.as<unsigned short>(), pk.Al.as<unsigned short>(), K, 0};
-    gemm16<true, true>(st, (int)P, (int)P, (int)K, n_omega, 1, A, A, d32, K, split, true, epi);
+    Opnd<unsigned short> A{pk.Ah.as<unsigned short>(), pk.Al.as<unsigned short>(), Kp, 0};
+    gemm16<true, true>(st, (int)P, (int)P, (int)Kp, n_omega, 1, A, A, d32, Kp, split, true, epi);
   }
   XG_HIP(hipStreamSynchronize(st));
 }
