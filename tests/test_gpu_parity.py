@@ -391,7 +391,11 @@ def test_davidson_single_root(ctx):
     assert abs(ev[0] - w[0]) < 1e-10
 
 
-def test_davidson_fp64_matches_dense(ctx):
+@pytest.mark.parametrize("gs", ["block", "column"])
+def test_davidson_fp64_matches_dense(ctx, gs, monkeypatch):
+    """FP64 Davidson against the dense spectrum, with the block (projection + Cholesky QR) and the
+    column-by-column Gram-Schmidt."""
+    monkeypatch.setenv("XTPGPU_GS", gs)
     M, eps, Hqp, homo, rpamin, vmin, cmax = random_bse_case(20, vt=8, ct=40, P=60, scale=0.05)
     t = dev_mmn(ctx, M)
     op = px.SingletOperator_TDA(eps, t, Hqp).configure(px.BSEOperator_Options(homo, rpamin, 0, vmin, cmax))
@@ -598,7 +602,9 @@ def test_setup_screening_gpu(ctx):
     assert relerr(G_ * sgn[None, None, :], Mr) < 1e-10
 
 
-def test_full_bse_davidson_matches_dense(ctx):
+@pytest.mark.parametrize("gs", ["block", "column"])
+def test_full_bse_davidson_matches_dense(ctx, gs, monkeypatch):
+    monkeypatch.setenv("XTPGPU_GS", gs)
     M, eps, Hqp, homo, rpamin, vmin, cmax = random_bse_case(40, vt=6, ct=30, P=50, scale=0.05)
     t = dev_mmn(ctx, M)
     opt = px.BSEOperator_Options(homo, rpamin, 0, vmin, cmax)
