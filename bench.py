@@ -168,12 +168,14 @@ def run_segments(args):
            "precision": prec}
     log(f"[segments] eps batch {t_batch*1e3:.1f} ms ({nseg/t_batch:.1f} seg/s) vs serial {t_serial*1e3:.1f} ms")
     del batch, rpas, bufs
-    # BSE-TDA, 8 roots per segment (full tensors, one segment at a time)
-    lat_bse = []
+    # BSE-TDA, 8 roots per segment (full tensors): one segment at a time, then the same segments
+    # spread over `--bse-streams` host threads, each with its own context (stream, BLAS handle)
     nb = min(nseg, args.bse_segments)
-    for s, (o, v, P) in enumerate(segs[:nb]):
+
+    def bse_segment(c, s):
+        o, v, P = segs[s]
         t0 = time.perf_counter()
-        t = px.TCMatrix_gwbse(ctx).Initialize(P, 0, o + v - 1, 0, o + v - 1)
+        t = px.TCMatrix_gwbse(c).Initialize(P, 0, o + v - 1, 0, o + v - 1)
         t.fill_synthetic(args.seed * 100003 + s, args.sigma0, 3.0)
         e = synthetic_energies(args.seed * 100003 + s, o, v)
         ei = px.setup_screening(t, e, o - 1, 0, o + v - 1)
@@ -182,13 +184,45 @@ def run_segments(args):
         ds.set_tolerance(1e-5)
         ds.set_iter_max(100)
         ds.set_max_search_space(80)
-        ds.solve(op, 8, want_vectors=False)
-        lat_bse.append(time.perf_counter() - t0)
+        ev, _ = ds.solve(op, 8, want_vectors=False)
         op.close()
         t.close()
+        return time.perf_counter() - t0, ev
+
+    t_all = time.perf_counter()
+    serial = [bse_segment(ctx, s) for s in range(nb)]
+    t_serial_bse = time.perf_counter() - t_all
+    lat_bse = [x[0] for x in serial]
     res.update({"bse_segments": nb, "bse_latency_ms_p50": float(np.percentile(lat_bse, 50) * 1e3),
                 "bse_latency_ms_p99": float(np.percentile(lat_bse, 99) * 1e3),
-                "bse_segments_per_s": nb / float(np.sum(lat_bse))})
+                "bse_segments_per_s": nb / t_serial_bse})
+    log(f"[segments] BSE-TDA serial: {nb} segments in {t_serial_bse:.2f} s")
+    if args.bse_streams > 1:
+        import threading
+        from concurrent.futures import ThreadPoolExecutor
+        local = threading.local()
+        ctxs = []
+
+        def work(s):
+            if not hasattr(local, "ctx"):
+                local.ctx = px.Context(0)
+                ctxs.append(local.ctx)
+            return bse_segment(local.ctx, s)
+
+        t_all = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=args.bse_streams) as pool:
+            conc = list(pool.map(work, range(nb)))
+        t_conc = time.perf_counter() - t_all
+        same = max(float(np.abs(a[1] - b[1]).max()) for a, b in zip(serial, conc))
+        lat_c = [x[0] for x in conc]
+        res.update({"bse_streams": args.bse_streams, "bse_concurrent_segments_per_s": nb / t_conc,
+                    "bse_concurrent_latency_ms_p50": float(np.percentile(lat_c, 50) * 1e3),
+                    "bse_concurrent_latency_ms_p99": float(np.percentile(lat_c, 99) * 1e3),
+                    "bse_concurrent_max_abs_diff_vs_serial": same})
+        log(f"[segments] BSE-TDA on {args.bse_streams} streams: {t_conc:.2f} s ({nb / t_conc:.1f} seg/s), "
+            f"max |dOmega| vs serial {same:.1e}")
+        for c in ctxs:
+            c.close()
     print(json.dumps({"workload": f"{nseg} uneven synthetic segments, n_occ U[10,60], n_virt n_occ*U[5,8], n_aux 4(n_occ+n_virt)", "n_segments": nseg, **res}))
 
 
@@ -452,6 +486,7 @@ def main():
                     help="matvec workload: also run a 1 GiB copy kernel (counter calibration)")
     ap.add_argument("--nseg", type=int, default=256)
     ap.add_argument("--bse-segments", type=int, default=32)
+    ap.add_argument("--bse-streams", type=int, default=4, help="segments workload: concurrent BSE solves")
     ap.add_argument("--segments-precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--skip-fp64-solve", action="store_true")
     args = ap.parse_args()
