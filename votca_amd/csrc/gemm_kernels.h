@@ -290,9 +290,9 @@ __device__ __forceinline__ void gemm_f64_tile(const GemmDims& d, const Opnd<doub
   __syncthreads();  // LDS is reused by the next tile of a persistent caller
 }
 
-template <int BM, int BN, bool AK, bool BK_, bool HAS_S, bool VEC, class Epi>
-__global__ __launch_bounds__(256) void gemm_f64_kernel(GemmDims d, Opnd<double> A, Opnd<double> B,
-                                                       const double* __restrict__ s, long s_bs, Epi epi) {
+template <int BM, int BN, bool AK, bool BK_, bool HAS_S, bool VEC, class Epi, int MINW = 1>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MINW))) void gemm_f64_kernel(
+    GemmDims d, Opnd<double> A, Opnd<double> B, const double* __restrict__ s, long s_bs, Epi epi) {
   int tm, tn;
   tile_of_block(d, tm, tn);
   gemm_f64_tile<BM, BN, AK, BK_, HAS_S, VEC, Epi>(d, A, B, s, s_bs, epi, tm, tn, blockIdx.z, blockIdx.y);
@@ -578,16 +578,16 @@ inline int gemm_tiles(const GemmDims& d) {
   return d.lower ? d.tiles_m * (d.tiles_m + 1) / 2 : d.tiles_m * d.tiles_n;
 }
 
-template <int BM, int BN, bool AK, bool BK_, class Epi>
+template <int BM, int BN, bool AK, bool BK_, class Epi, int MINW = 1>
 inline hipError_t launch_gemm_f64(hipStream_t st, const GemmDims& d, Opnd<double> A, Opnd<double> B,
                                   const double* s, long s_bs, Epi epi) {
   dim3 grid(gemm_tiles(d), d.ksplit, d.batch);
   const bool vec = AK && BK_ && vec_ok(A, d.K, 2) && vec_ok(B, d.K, 2) && (s_bs % 2 == 0);
   if (vec) {
     if (s)
-      hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, AK, BK_, true, true, Epi>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+      hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, AK, BK_, true, true, Epi, MINW>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
     else
-      hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, AK, BK_, false, true, Epi>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
+      hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, AK, BK_, false, true, Epi, MINW>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);
   } else {
     if (s)
       hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, AK, BK_, true, false, Epi>), grid, dim3(256), 0, st, d, A, B, s, s_bs, epi);

@@ -179,7 +179,13 @@ void gemm64(hipStream_t st, int M, int N, int K, int batch, int ksplit, Opnd<dou
   if (ksplit <= 1 && M >= 256 && N >= 256 && big_tiles >= 512) {
     GemmDims d = make_dims<128, 128>(M, N, std::max(K, 0), batch, ksplit, 16, lower);
     if (!lower) d.group = tile_group(d);
-    XG_HIP((launch_gemm_f64<128, 128, AK, BK_>(st, d, A, B, s, s_bs, epi)));
+    // registers capped for two waves per SIMD (210 VGPRs, accumulators in VGPRs): 61 against 44
+    // TFLOP/s at one wave per SIMD on the Hd stage-1 product; XTPGPU_F64_KERNEL=occ1 selects the latter
+    const char* e = std::getenv("XTPGPU_F64_KERNEL");
+    if (e && std::strcmp(e, "occ1") == 0)
+      XG_HIP((launch_gemm_f64<128, 128, AK, BK_>(st, d, A, B, s, s_bs, epi)));
+    else
+      XG_HIP((launch_gemm_f64<128, 128, AK, BK_, Epi, 2>(st, d, A, B, s, s_bs, epi)));
     return;
   }
   GemmDims d = make_dims<64, 64>(M, N, std::max(K, 0), batch, ksplit, 16, lower);
