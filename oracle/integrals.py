@@ -1,4 +1,5 @@
-"""Gaussian integrals (overlap, 2- and 3-centre Coulomb) for s/p basis sets.
+"""Gaussian integrals (overlap, 2- and 3-centre Coulomb, kinetic, nuclear attraction) for s/p
+basis sets.
 
 TEST INFRASTRUCTURE ONLY: part of the oracle. The reference builds its three-centre
 tensor with libint2 (xtp/src/libxtp/libint2_calls.cc:540-651), which is not present
@@ -17,7 +18,9 @@ Conventions matched to the reference:
     (xtp/src/libxtp/aoshell.cc:118-131);
   * geometry read in Angstrom and converted with ang2bohr
     (tools/include/votca/tools/constants.h:48-54).
-The result is pinned against threecenter_gwbse/ref{0,2,4}b.mm in the tests.
+The result is pinned against threecenter_gwbse/ref{0,2,4}b.mm in the tests; the one-electron
+integrals by closed forms (one s primitive, the hydrogen variational bound) and, through the
+core-Hamiltonian MOs they produce, by the plasmon-pole known answers of test_ppm.cc.
 """
 from __future__ import annotations
 
