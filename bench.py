@@ -179,11 +179,15 @@ def run_segments(args):
         t.fill_synthetic(args.seed * 100003 + s, args.sigma0, 3.0)
         e = synthetic_energies(args.seed * 100003 + s, o, v)
         ei = px.setup_screening(t, e, o - 1, 0, o + v - 1)
-        op = px.SingletOperator_TDA(ei, t, np.diag(e)).configure(px.BSEOperator_Options(o - 1, 0, 0, 0, o + v - 1))
+        precs = ("fp64",) if prec == "fp64" else ("fp64", prec)
+        op = px.SingletOperator_TDA(ei, t, np.diag(e), precisions=precs).configure(
+            px.BSEOperator_Options(o - 1, 0, 0, 0, o + v - 1))
         ds = px.DavidsonSolver()
         ds.set_tolerance(1e-5)
         ds.set_iter_max(100)
         ds.set_max_search_space(80)
+        if prec != "fp64":  # mixed iterations, FP64 refinement to the same 1e-5 residual
+            ds.set_precision(prec, refine_tolerance=1e-5, refine_max=8)
         ev, _ = ds.solve(op, 8, want_vectors=False)
         op.close()
         t.close()
