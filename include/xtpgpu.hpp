@@ -4,6 +4,8 @@
 //   xtpgpu::BSEOperator  ~ BSE_OPERATOR<...>     (xtp/include/votca/xtp/bse_operator.h:40-87),
 //                          a MatrixFreeOperator (rows(), diagonal(), matmul()) as the solver needs
 //                          (xtp/include/votca/xtp/matrixfreeoperator.h:45-70)
+//   xtpgpu::RPA          ~ RPA                   (xtp/include/votca/xtp/rpa.h:35-115)
+//   xtpgpu::DavidsonSolver ~ DavidsonSolver      (xtp/include/votca/xtp/davidsonsolver.h:47-334)
 // Errors come back as std::runtime_error with the library's message, as the reference throws.
 // Matrices are plain column-major double buffers (Eigen::MatrixXd::data() fits directly).
 #ifndef XTPGPU_HPP
@@ -89,6 +91,138 @@ class BSEOperator {
  private:
   xtpgpu_bseop h_ = nullptr;
   int64_t size_ = 0;
+};
+
+// RPA (xtp/include/votca/xtp/rpa.h:35-115): dielectric builds and the two-particle Hamiltonian.
+class RPA {
+ public:
+  RPA(Context& ctx, TCMatrix& Mmn) : ctx_(ctx.get()), mmn_(Mmn.get()), aux_(Mmn.auxsize()) {}
+  void configure(int64_t homo, int64_t rpamin, int64_t rpamax) {
+    homo_ = homo;
+    rpamin_ = rpamin;
+    rpamax_ = rpamax;
+  }
+  void setRPAInputEnergies(const std::vector<double>& e) { energies_ = e; }
+  // eps(i w) (aux x aux, column-major) at one imaginary frequency (rpa.cc:75-140)
+  std::vector<double> calculate_epsilon_i(double frequency, int precision = XTPGPU_FP64) const {
+    return epsilon(XTPGPU_EPS_IMAG, frequency, 0.0, precision);
+  }
+  // eps at a real frequency (rpa.cc:124-132) or a complex one (rpa.cc:145-202)
+  std::vector<double> calculate_epsilon_r(double frequency, int precision = XTPGPU_FP64) const {
+    return epsilon(XTPGPU_EPS_REAL, frequency, 0.0, precision);
+  }
+  std::vector<double> calculate_epsilon_r(double re, double im, int precision = XTPGPU_FP64) const {
+    return epsilon(XTPGPU_EPS_COMPLEX, re, im, precision);
+  }
+  struct rpa_eigensolution {  // rpa.h:78-82
+    std::vector<double> omega, XpY;
+    double ERPA_correlation = 0.0;
+  };
+  // RPA::Diagonalize_H2p (rpa.cc:204-258)
+  rpa_eigensolution Diagonalize_H2p() const {
+    const int64_t n = (homo_ + 1 - rpamin_) * (rpamax_ - homo_);
+    rpa_eigensolution sol;
+    sol.omega.resize(n);
+    sol.XpY.resize(n * n);
+    check(xtpgpu_rpa_h2p(ctx_, mmn_, energies_.data(), homo_, rpamin_, rpamax_, sol.omega.data(), sol.XpY.data(),
+                         &sol.ERPA_correlation));
+    return sol;
+  }
+
+ private:
+  std::vector<double> epsilon(int kind, double re, double im, int precision) const {
+    std::vector<double> out(aux_ * aux_);
+    check(xtpgpu_rpa_epsilon(ctx_, mmn_, energies_.data(), homo_, rpamin_, rpamax_, kind, &re, &im, 1, 1e-4,
+                             precision, out.data()));
+    return out;
+  }
+  xtpgpu_ctx ctx_;
+  xtpgpu_mmn mmn_;
+  int64_t aux_, homo_ = 0, rpamin_ = 0, rpamax_ = 0;
+  std::vector<double> energies_;
+};
+
+// DavidsonSolver (xtp/include/votca/xtp/davidsonsolver.h:47-334) with the device-resident loop:
+// the same options and string values, eigenvalues()/eigenvectors() after solve().
+class DavidsonSolver {
+ public:
+  DavidsonSolver() {
+    o_.tolerance = 1e-4;
+    o_.correction = 0;
+    o_.size_update = 1;
+    o_.iter_max = 50;
+    o_.max_search_space = 0;
+    o_.precision = XTPGPU_FP64;
+    o_.refine_tolerance = 1e-4;
+    o_.refine_max = 0;
+    o_.precision_start = -1;
+    o_.switch_tolerance = 0.0;
+  }
+  void set_iter_max(int n) { o_.iter_max = n; }
+  void set_max_search_space(int n) { o_.max_search_space = n; }
+  void set_tolerance(double t) { o_.tolerance = t; }
+  void set_tolerance(const std::string& t) {  // davidsonsolver.cc:122-134
+    if (t == "loose") o_.tolerance = 1e-3;
+    else if (t == "normal") o_.tolerance = 1e-4;
+    else if (t == "strict") o_.tolerance = 1e-5;
+    else if (t == "lapack") o_.tolerance = 1e-9;
+    else throw std::runtime_error(t + " is not a valid Davidson tolerance");
+  }
+  void set_correction(const std::string& m) {  // davidsonsolver.cc:111-120
+    if (m == "DPR") o_.correction = 0;
+    else if (m == "OLSEN") o_.correction = 1;
+    else throw std::runtime_error(m + " is not a valid Davidson correction method");
+  }
+  void set_size_update(const std::string& u) {  // davidsonsolver.cc:136-147
+    if (u == "min") o_.size_update = 0;
+    else if (u == "safe") o_.size_update = 1;
+    else if (u == "max") o_.size_update = 2;
+    else throw std::runtime_error(u + " is not a valid Davidson update");
+  }
+  // operator precision of the iterations (XTPGPU_BF16 / XTPGPU_BF16X3 add FP64 refinement rounds)
+  void set_precision(int precision, double refine_tolerance, int refine_max = 4, int start = -1,
+                     double switch_tolerance = 1e-5) {
+    o_.precision = precision;
+    o_.refine_tolerance = refine_tolerance;
+    o_.refine_max = precision == XTPGPU_FP64 ? 0 : refine_max;
+    o_.precision_start = start;
+    o_.switch_tolerance = switch_tolerance;
+  }
+  // SYMM solve of a BSE operator; returns false when not converged (unconverged roots zeroed,
+  // as storeNotConvergedData does)
+  template <class Op>
+  bool solve(const Op& A, int64_t neigen) {
+    n_ = A.rows();
+    eval_.assign(neigen, 0.0);
+    evec_.assign(n_ * neigen, 0.0);
+    return finish(xtpgpu_davidson_solve(A.get(), neigen, &o_, eval_.data(), evec_.data(), n_, &info_));
+  }
+  // full BSE through the HamiltonianOperator [[A, B], [-B, -A]] (BSE::Solve_nonhermitian_Davidson)
+  template <class OpA, class OpB>
+  bool solve_full(const OpA& A, const OpB& B, int64_t neigen) {
+    n_ = A.rows();
+    eval_.assign(neigen, 0.0);
+    evec_.assign(n_ * neigen, 0.0);
+    evec2_.assign(n_ * neigen, 0.0);
+    return finish(xtpgpu_davidson_solve_full(A.get(), B.get(), neigen, &o_, eval_.data(), evec_.data(),
+                                             evec2_.data(), n_, &info_));
+  }
+  const std::vector<double>& eigenvalues() const { return eval_; }
+  const std::vector<double>& eigenvectors() const { return evec_; }   // size x neigen
+  const std::vector<double>& eigenvectors2() const { return evec2_; } // Y of the full BSE
+  int num_iterations() const { return info_.iterations; }
+  const xtpgpu_davidson_info& info() const { return info_; }
+
+ private:
+  bool finish(int rc) {
+    if (rc == XTPGPU_ERR_NOT_CONVERGED) return false;
+    check(rc);
+    return true;
+  }
+  xtpgpu_davidson_opts o_{};
+  xtpgpu_davidson_info info_{};
+  int64_t n_ = 0;
+  std::vector<double> eval_, evec_, evec2_;
 };
 
 using SingletOperator_TDA = BSEOperator<1, 2, 1, 0>;

@@ -50,7 +50,14 @@ def test_cpp_header_compiles_and_links(tmp_path):
         "int main(int argc, char**) {\n"
         "  if (argc > 5) { xtpgpu::Context c(0); xtpgpu::TCMatrix t(c, 4, 0, 3, 0, 3);\n"
         "    xtpgpu::BSEOperatorOptions o{1, 0, 0, 0, 3};\n"
-        "    xtpgpu::SingletOperator_TDA H(c, t, nullptr, nullptr, 4, o); return (int)H.rows(); }\n"
+        "    xtpgpu::SingletOperator_TDA H(c, t, nullptr, nullptr, 4, o);\n"
+        "    xtpgpu::SingletOperator_BTDA_B Bop(c, t, nullptr, nullptr, 4, o);\n"
+        "    xtpgpu::RPA rpa(c, t); rpa.configure(1, 0, 3); rpa.setRPAInputEnergies({-1, -0.5, 0.2, 0.4});\n"
+        "    auto eps = rpa.calculate_epsilon_i(0.5); auto h2p = rpa.Diagonalize_H2p();\n"
+        "    xtpgpu::DavidsonSolver ds; ds.set_tolerance(\"lapack\"); ds.set_correction(\"DPR\");\n"
+        "    ds.set_size_update(\"safe\"); ds.set_precision(XTPGPU_BF16X3, 1e-7, 8, XTPGPU_BF16);\n"
+        "    bool ok = ds.solve(H, 2) && ds.solve_full(H, Bop, 2);\n"
+        "    return (int)H.rows() + (int)eps.size() + (int)h2p.omega.size() + (ok ? 1 : 0); }\n"
         "  return 0; }\n")
     out = tmp_path / "use"
     r = subprocess.run([gxx, "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(out),

@@ -13,6 +13,7 @@ from votca_amd import pyxtp as px
 
 pytestmark = pytest.mark.gpu
 G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EPS_INV_BSE_OP = np.array([0.999807798016267, 0.994206065211371, 0.917916768047073,
     0.902913813951883, 0.902913745974602, 0.902913584797742, 0.853352878674581, 0.853352727016914,
     0.853352541699637, 0.79703468058566, 0.797034577207669, 0.797034400395582, 0.787701833916331,
@@ -765,3 +766,79 @@ def test_eps_batch_matches_per_segment(ctx, prec, tol):
         b.run(use_graph)
         for s_ in range(5):
             assert relerr(b.result(s_), refs[s_]) < tol
+
+
+CPP_DRIVER = r"""
+#include "xtpgpu.hpp"
+#include <cstdio>
+#include <fstream>
+#include <vector>
+// reads: ntot P vt ct, M[m][n][P] (m-major), eps[P], Hqp[(vt+ct)^2] column-major; prints 5 + 4 roots
+int main(int argc, char** argv) {
+  std::ifstream in(argv[1], std::ios::binary);
+  int64_t hdr[4];
+  in.read(reinterpret_cast<char*>(hdr), sizeof(hdr));
+  const int64_t ntot = hdr[0], P = hdr[1], vt = hdr[2], ct = hdr[3], hs = vt + ct;
+  std::vector<double> M(ntot * ntot * P), eps(P), H(hs * hs);
+  in.read(reinterpret_cast<char*>(M.data()), M.size() * 8);
+  in.read(reinterpret_cast<char*>(eps.data()), eps.size() * 8);
+  in.read(reinterpret_cast<char*>(H.data()), H.size() * 8);
+  xtpgpu::Context c(0);
+  xtpgpu::TCMatrix t(c, P, 0, ntot - 1, 0, ntot - 1);
+  std::vector<double> slice(ntot * P);
+  for (int64_t m = 0; m < ntot; ++m) {
+    for (int64_t n = 0; n < ntot; ++n)
+      for (int64_t p = 0; p < P; ++p) slice[p * ntot + n] = M[(m * ntot + n) * P + p];
+    t.set_slice(m, slice.data(), ntot);
+  }
+  xtpgpu::BSEOperatorOptions o{vt - 1, 0, 0, 0, hs - 1};
+  xtpgpu::SingletOperator_TDA A(c, t, eps.data(), H.data(), hs, o);
+  xtpgpu::SingletOperator_BTDA_B B(c, t, eps.data(), H.data(), hs, o);
+  xtpgpu::DavidsonSolver ds;
+  ds.set_tolerance(1e-9);
+  ds.set_max_search_space(50);
+  if (!ds.solve(A, 5)) return 2;
+  for (double e : ds.eigenvalues()) std::printf("%.17g\n", e);
+  xtpgpu::DavidsonSolver df;
+  df.set_tolerance(1e-8);
+  df.set_max_search_space(40);
+  df.set_iter_max(100);
+  if (!df.solve_full(A, B, 4)) return 3;
+  for (double e : df.eigenvalues()) std::printf("%.17g\n", e);
+  return 0;
+}
+"""
+
+
+def test_cpp_wrapper_davidson_gpu(tmp_path):
+    """include/xtpgpu.hpp driven from a C++ program (as VOTCA-XTP would link it): TCMatrix filled
+    slice by slice, SingletOperator_TDA / _BTDA_B, DavidsonSolver.solve and .solve_full; the roots
+    against the dense TDA and full-BSE spectra of the oracle's matrices."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("no g++")
+    M, eps, Hqp, homo, rpamin, vmin, cmax = random_bse_case(40, vt=6, ct=30, P=50, scale=0.05)
+    ntot, P = M.shape[0], M.shape[2]
+    f = tmp_path / "in.bin"
+    with open(f, "wb") as fh:
+        fh.write(np.array([ntot, P, 6, 30], dtype=np.int64).tobytes())
+        fh.write(np.ascontiguousarray(M).tobytes())
+        fh.write(np.ascontiguousarray(eps, dtype=np.float64).tobytes())
+        fh.write(np.asfortranarray(Hqp).tobytes(order="F"))
+    src = tmp_path / "drv.cc"
+    src.write_text(CPP_DRIVER)
+    exe = tmp_path / "drv"
+    libdir = os.path.dirname(px.LIB_PATH)
+    r = subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+                        "-L", libdir, "-lxtpgpu", "-Wl,-rpath," + libdir], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    print("[cpp-wrapper] running the C++ driver", flush=True)
+    r = subprocess.run([str(exe), str(f)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, (r.returncode, r.stderr)
+    vals = np.array([float(x) for x in r.stdout.split()])
+    Ad = X.bse_dense(1, 2, 1, 0, eps, M, Hqp, homo, rpamin, vmin, cmax)
+    Bd = X.bse_dense(0, 2, 0, 1, eps, M, Hqp, homo, rpamin, vmin, cmax)
+    assert np.abs(vals[:5] - np.linalg.eigvalsh(Ad)[:5]).max() < 1e-8
+    w = np.sort(np.linalg.eigvals(np.block([[Ad, Bd], [-Bd, -Ad]])).real)
+    assert np.abs(vals[5:] - w[w > 0][:4]).max() < 1e-7
