@@ -6,6 +6,7 @@
 //                          (xtp/include/votca/xtp/matrixfreeoperator.h:45-70)
 //   xtpgpu::RPA          ~ RPA                   (xtp/include/votca/xtp/rpa.h:35-115)
 //   xtpgpu::DavidsonSolver ~ DavidsonSolver      (xtp/include/votca/xtp/davidsonsolver.h:47-334)
+//   xtpgpu::Sigma_PPM    ~ Sigma_PPM + GW::SolveQP (sigma_ppm.h, gw.cc:323-410)
 // Errors come back as std::runtime_error with the library's message, as the reference throws.
 // Matrices are plain column-major double buffers (Eigen::MatrixXd::data() fits directly).
 #ifndef XTPGPU_HPP
@@ -223,6 +224,82 @@ class DavidsonSolver {
   xtpgpu_davidson_info info_{};
   int64_t n_ = 0;
   std::vector<double> eval_, evec_, evec2_;
+};
+
+// Sigma_PPM (xtp/src/libxtp/self_energy_evaluators/sigma_ppm.h) with the exchange of Sigma_base
+// (sigma_base.h:33-106) and GW::SolveQP (gw.cc:323-410) on the device. PrepareScreening rotates
+// the tensor in place (Mmn * phi), as the reference does.
+class Sigma_PPM {
+ public:
+  Sigma_PPM(Context& ctx, TCMatrix& Mmn) : ctx_(ctx.get()), mmn_(Mmn.get()), aux_(Mmn.auxsize()) {}
+  ~Sigma_PPM() { xtpgpu_sigma_destroy(h_); }
+  Sigma_PPM(const Sigma_PPM&) = delete;
+  Sigma_PPM& operator=(const Sigma_PPM&) = delete;
+  void configure(int64_t homo, int64_t qpmin, int64_t qpmax, int64_t rpamin, int64_t rpamax, double eta) {
+    homo_ = homo;
+    qpmin_ = qpmin;
+    qpmax_ = qpmax;
+    rpamin_ = rpamin;
+    rpamax_ = rpamax;
+    eta_ = eta;
+  }
+  void setRPAInputEnergies(const std::vector<double>& e) { energies_ = e; }
+  // Sigma_base::CalcExchangeMatrix (sigma_base.cc:36-52): qptotal x qptotal, column-major
+  std::vector<double> CalcExchangeMatrix() const {
+    const int64_t q = qpmax_ - qpmin_ + 1;
+    std::vector<double> out(q * q);
+    check(xtpgpu_sigma_x(ctx_, mmn_, homo_, rpamin_, qpmin_, qpmax_, out.data()));
+    return out;
+  }
+  // PPM::PPM_construct_parameters + Sigma_PPM::PrepareScreening (ppm.cc:31-59, sigma_ppm.cc:32-35)
+  void PrepareScreening() {
+    weight_.resize(aux_);
+    freq_.resize(aux_);
+    check(xtpgpu_ppm_prepare(ctx_, mmn_, energies_.data(), homo_, rpamin_, rpamax_, weight_.data(), freq_.data()));
+    xtpgpu_sigma_destroy(h_);
+    h_ = nullptr;
+    check(xtpgpu_sigma_ppm_create(ctx_, mmn_, energies_.data(), weight_.data(), freq_.data(), homo_, rpamin_, qpmin_,
+                                  eta_, &h_));
+  }
+  const std::vector<double>& getPpm_weight() const { return weight_; }
+  const std::vector<double>& getPpm_freq() const { return freq_; }
+  // CalcCorrelationDiagElement (+Derivative) for every QP level at its own frequency
+  std::vector<double> CalcCorrelationDiag(const std::vector<double>& frequencies,
+                                          std::vector<double>* derivative = nullptr) const {
+    const int64_t q = (int64_t)frequencies.size();
+    std::vector<int64_t> lv(q);
+    for (int64_t l = 0; l < q; ++l) lv[l] = l;
+    std::vector<double> out(q);
+    if (derivative) derivative->resize(q);
+    check(xtpgpu_sigma_ppm_diag(h_, q, lv.data(), frequencies.data(), out.data(),
+                                derivative ? derivative->data() : nullptr));
+    return out;
+  }
+  // Sigma_base::CalcCorrelationOffDiag (sigma_base.cc:65-78); the diagonal is left zero
+  std::vector<double> CalcCorrelationOffDiag(const std::vector<double>& frequencies) const {
+    const int64_t q = (int64_t)frequencies.size();
+    std::vector<double> out(q * q, 0.0);
+    check(xtpgpu_sigma_ppm_offdiag(h_, q, frequencies.data(), out.data()));
+    return out;
+  }
+  // GW::SolveQP: w_l = intercept_l + Sigma_c(l, w_l) for all QP levels
+  std::vector<double> SolveQP(const std::vector<double>& intercepts, const std::vector<double>& frequencies,
+                              const xtpgpu_qp_opts& opts, std::vector<int>* converged = nullptr) const {
+    const int64_t q = (int64_t)intercepts.size();
+    std::vector<double> out(q);
+    std::vector<int> conv(q);
+    check(xtpgpu_gw_solve_qp(h_, q, intercepts.data(), frequencies.data(), &opts, out.data(), conv.data()));
+    if (converged) *converged = conv;
+    return out;
+  }
+
+ private:
+  xtpgpu_ctx ctx_;
+  xtpgpu_mmn mmn_;
+  int64_t aux_, homo_ = 0, qpmin_ = 0, qpmax_ = 0, rpamin_ = 0, rpamax_ = 0;
+  double eta_ = 1e-3;
+  std::vector<double> energies_, weight_, freq_;
+  xtpgpu_sigma h_ = nullptr;
 };
 
 using SingletOperator_TDA = BSEOperator<1, 2, 1, 0>;

@@ -842,3 +842,71 @@ def test_cpp_wrapper_davidson_gpu(tmp_path):
     assert np.abs(vals[:5] - np.linalg.eigvalsh(Ad)[:5]).max() < 1e-8
     w = np.sort(np.linalg.eigvals(np.block([[Ad, Bd], [-Bd, -Ad]])).real)
     assert np.abs(vals[5:] - w[w > 0][:4]).max() < 1e-7
+
+
+CPP_SIGMA_DRIVER = r"""
+#include "xtpgpu.hpp"
+#include <cstdio>
+#include <fstream>
+#include <vector>
+// reads: ntot P homo, M[m][n][P] (m-major), e[ntot]; prints Sigma_x then Sigma_c (ntot^2 each, column-major)
+int main(int argc, char** argv) {
+  std::ifstream in(argv[1], std::ios::binary);
+  int64_t hdr[3];
+  in.read(reinterpret_cast<char*>(hdr), sizeof(hdr));
+  const int64_t ntot = hdr[0], P = hdr[1], homo = hdr[2];
+  std::vector<double> M(ntot * ntot * P), e(ntot);
+  in.read(reinterpret_cast<char*>(M.data()), M.size() * 8);
+  in.read(reinterpret_cast<char*>(e.data()), e.size() * 8);
+  xtpgpu::Context c(0);
+  xtpgpu::TCMatrix t(c, P, 0, ntot - 1, 0, ntot - 1);
+  std::vector<double> slice(ntot * P);
+  for (int64_t m = 0; m < ntot; ++m) {
+    for (int64_t n = 0; n < ntot; ++n)
+      for (int64_t p = 0; p < P; ++p) slice[p * ntot + n] = M[(m * ntot + n) * P + p];
+    t.set_slice(m, slice.data(), ntot);
+  }
+  xtpgpu::Sigma_PPM s(c, t);
+  s.configure(homo, 0, ntot - 1, 0, ntot - 1, 1e-3);
+  s.setRPAInputEnergies(e);
+  std::vector<double> x = s.CalcExchangeMatrix();
+  s.PrepareScreening();
+  std::vector<double> cmat = s.CalcCorrelationOffDiag(e);
+  std::vector<double> d = s.CalcCorrelationDiag(e);
+  for (int64_t l = 0; l < ntot; ++l) cmat[l * ntot + l] = d[l];
+  for (double v : x) std::printf("%.17g\n", v);
+  for (double v : cmat) std::printf("%.17g\n", v);
+  return 0;
+}
+"""
+
+
+def test_cpp_wrapper_sigma_ppm_gpu(tmp_path, ints):  # test_sigma_ppm.cc:44-121 through include/xtpgpu.hpp
+    """xtpgpu::Sigma_PPM from a C++ program: exchange matrix, PPM screening and the correlation
+    matrix against the sigma_ppm fixtures (isApprox 1e-5)."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("no g++")
+    S, Saux, V, T = ints
+    M = X.fill_mmn(T, mmio.read_matrix(os.path.join(G, "sigma_ppm/MOs.mm")), 0, 16, 0, 16, Saux, V)
+    f = tmp_path / "in.bin"
+    with open(f, "wb") as fh:
+        fh.write(np.array([17, M.shape[2], 4], dtype=np.int64).tobytes())
+        fh.write(np.ascontiguousarray(M).tobytes())
+        fh.write(np.ascontiguousarray(SIGMA_PPM_E, dtype=np.float64).tobytes())
+    src = tmp_path / "sig.cc"
+    src.write_text(CPP_SIGMA_DRIVER)
+    exe = tmp_path / "sig"
+    libdir = os.path.dirname(px.LIB_PATH)
+    r = subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+                        "-L", libdir, "-lxtpgpu", "-Wl,-rpath," + libdir], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    print("[cpp-wrapper] running the Sigma_PPM driver", flush=True)
+    r = subprocess.run([str(exe), str(f)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, (r.returncode, r.stderr)
+    vals = np.array([float(v) for v in r.stdout.split()])
+    x = vals[:289].reshape(17, 17, order="F")
+    c = vals[289:].reshape(17, 17, order="F")
+    assert mmio.isapprox(x, mmio.read_matrix(os.path.join(G, "sigma_ppm/x_ref.mm")), 1e-5)
+    assert mmio.isapprox(c, mmio.read_matrix(os.path.join(G, "sigma_ppm/c_ref.mm")), 1e-5)
