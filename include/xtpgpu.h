@@ -135,6 +135,14 @@ int xtpgpu_bse_create(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* eps0inv, con
                       int64_t cmax, int cqp, int cx, int cd, int cd2, unsigned precision_mask,
                       xtpgpu_bseop* out);
 int xtpgpu_bse_destroy(xtpgpu_bseop op);
+/* BSE::AdjustHqpSize (bse.cc:147-186), host only: the GW-window QP Hamiltonian Hqp (gw x gw,
+ * gw = qpmax - qpmin + 1, leading dim ldh) cut or extended to the BSE window [vmin, cmax]
+ * (hs = (homo - vmin + 1) + (cmax - homo) levels): overlapping levels take their Hqp block, a BSE
+ * level outside the GW window gets its RPA input energy rpa_energies[level - rpamin] (n_rpa values)
+ * on the diagonal. out: hs x hs, leading dim ldo. */
+int xtpgpu_bse_adjust_hqp(const double* Hqp, int64_t ldh, const double* rpa_energies, int64_t n_rpa,
+                          int64_t homo, int64_t rpamin, int64_t qpmin, int64_t qpmax, int64_t vmin,
+                          int64_t cmax, double* out, int64_t ldo);
 int xtpgpu_bse_size(xtpgpu_bseop op, int64_t* size);
 /* BSE_OPERATOR::diagonal (bse_operator.cc:134-175). */
 int xtpgpu_bse_diagonal(xtpgpu_bseop op, double* out);
@@ -187,7 +195,9 @@ int xtpgpu_sigma_ppm_offdiag(xtpgpu_sigma s, int64_t qptotal, const double* freq
  * kappa0 exp(-(alpha x_j)^2) on the device -- with the frequency-independent contractions
  * S_{j,l,n} = (M_l D_j M_l^T)_nn and T_{l,n} = (M_l kappa0 M_l^T)_nn done once. points/weights are
  * the scaled quadrature nodes and weights (QuadratureFactory scheme and order), symmetry the
- * scheme's UseSymmetry(). The tensor must span the RPA window in n. */
+ * scheme's UseSymmetry(). The tensor must span the RPA window in n. eta is the GW option's
+ * broadening, kept for the option surface: Sigma_c^CDA itself uses the RPA's fixed eta (1e-4,
+ * rpa.h:90) in the residues and the imaginary-axis integrand, as sigma_cda.cc:107,125 do. */
 typedef struct xtpgpu_cda_s* xtpgpu_cda;
 int xtpgpu_sigma_cda_create(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* rpa_energies, int64_t homo,
                             int64_t rpamin, int64_t rpamax, int64_t qpmin, int64_t qpmax, double eta,

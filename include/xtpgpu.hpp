@@ -125,13 +125,20 @@ class RPA {
     rpa_eigensolution sol;
     sol.omega.resize(n);
     sol.XpY.resize(n * n);
+    check_energies();
     check(xtpgpu_rpa_h2p(ctx_, mmn_, energies_.data(), homo_, rpamin_, rpamax_, sol.omega.data(), sol.XpY.data(),
                          &sol.ERPA_correlation));
     return sol;
   }
 
  private:
+  // the library reads rpamax - rpamin + 1 input energies
+  void check_energies() const {
+    if ((int64_t)energies_.size() != rpamax_ - rpamin_ + 1)
+      throw std::runtime_error("RPA: RPA input energies do not match rpamax - rpamin + 1");
+  }
   std::vector<double> epsilon(int kind, double re, double im, int precision) const {
+    check_energies();
     std::vector<double> out(aux_ * aux_);
     check(xtpgpu_rpa_epsilon(ctx_, mmn_, energies_.data(), homo_, rpamin_, rpamax_, kind, &re, &im, 1, 1e-4,
                              precision, out.data()));
@@ -253,6 +260,8 @@ class Sigma_PPM {
   }
   // PPM::PPM_construct_parameters + Sigma_PPM::PrepareScreening (ppm.cc:31-59, sigma_ppm.cc:32-35)
   void PrepareScreening() {
+    if ((int64_t)energies_.size() != rpamax_ - rpamin_ + 1)
+      throw std::runtime_error("Sigma_PPM: RPA input energies do not match rpamax - rpamin + 1");
     weight_.resize(aux_);
     freq_.resize(aux_);
     check(xtpgpu_ppm_prepare(ctx_, mmn_, energies_.data(), homo_, rpamin_, rpamax_, weight_.data(), freq_.data()));
@@ -267,6 +276,7 @@ class Sigma_PPM {
   std::vector<double> CalcCorrelationDiag(const std::vector<double>& frequencies,
                                           std::vector<double>* derivative = nullptr) const {
     const int64_t q = (int64_t)frequencies.size();
+    if (q > qpmax_ - qpmin_ + 1) throw std::runtime_error("CalcCorrelationDiag: more frequencies than QP levels");
     std::vector<int64_t> lv(q);
     for (int64_t l = 0; l < q; ++l) lv[l] = l;
     std::vector<double> out(q);
@@ -278,6 +288,7 @@ class Sigma_PPM {
   // Sigma_base::CalcCorrelationOffDiag (sigma_base.cc:65-78); the diagonal is left zero
   std::vector<double> CalcCorrelationOffDiag(const std::vector<double>& frequencies) const {
     const int64_t q = (int64_t)frequencies.size();
+    if (q != qpmax_ - qpmin_ + 1) throw std::runtime_error("CalcCorrelationOffDiag: frequencies must have qptotal entries");
     std::vector<double> out(q * q, 0.0);
     check(xtpgpu_sigma_ppm_offdiag(h_, q, frequencies.data(), out.data()));
     return out;
@@ -286,6 +297,8 @@ class Sigma_PPM {
   std::vector<double> SolveQP(const std::vector<double>& intercepts, const std::vector<double>& frequencies,
                               const xtpgpu_qp_opts& opts, std::vector<int>* converged = nullptr) const {
     const int64_t q = (int64_t)intercepts.size();
+    if ((int64_t)frequencies.size() != q || q != qpmax_ - qpmin_ + 1)
+      throw std::runtime_error("SolveQP: intercepts and frequencies must both have qptotal entries");
     std::vector<double> out(q);
     std::vector<int> conv(q);
     check(xtpgpu_gw_solve_qp(h_, q, intercepts.data(), frequencies.data(), &opts, out.data(), conv.data()));
@@ -301,6 +314,19 @@ class Sigma_PPM {
   std::vector<double> energies_, weight_, freq_;
   xtpgpu_sigma h_ = nullptr;
 };
+
+// BSE::AdjustHqpSize (bse.cc:147-186): the GW-window Hqp (gw x gw, column-major) cut or extended to
+// the BSE window [vmin, cmax]; levels outside the GW window take their RPA input energy.
+inline std::vector<double> AdjustHqpSize(const std::vector<double>& Hqp, const std::vector<double>& rpa_energies,
+                                         int64_t homo, int64_t rpamin, int64_t qpmin, int64_t qpmax, int64_t vmin,
+                                         int64_t cmax) {
+  const int64_t gw = qpmax - qpmin + 1, hs = (homo - vmin + 1) + (cmax - homo);
+  if (gw <= 0 || (int64_t)Hqp.size() != gw * gw) throw std::runtime_error("AdjustHqpSize: Hqp is not qptotal x qptotal");
+  std::vector<double> out((size_t)hs * hs);
+  check(xtpgpu_bse_adjust_hqp(Hqp.data(), gw, rpa_energies.data(), (int64_t)rpa_energies.size(), homo, rpamin, qpmin,
+                              qpmax, vmin, cmax, out.data(), hs));
+  return out;
+}
 
 using SingletOperator_TDA = BSEOperator<1, 2, 1, 0>;
 using TripletOperator_TDA = BSEOperator<1, 0, 1, 0>;

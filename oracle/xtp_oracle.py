@@ -135,6 +135,26 @@ def rpa_epsilon(M: np.ndarray, energies: np.ndarray, homo: int, rpamin: int, rpa
 
 
 # ---------------------------------------------------------------- BSE operator
+def rpa_epsilon_columns(M, energies: np.ndarray, homo: int, rpamin: int, rpamax: int, freq, kind: str,
+                        cols, eta: float = 1e-4) -> np.ndarray:
+    """Columns `cols` of RPA::calculate_epsilon (xtp/src/libxtp/gwbse/rpa.cc:75-140): the per-level
+    accumulation eps += M_v^T diag(d_v) M_v of rpa.cc:88-136 (M_v = Mmn_[v].bottomRows(n_unocc))
+    restricted to the requested columns, plus the identity. M is indexable by slice (occupied
+    levels v - rpamin) and returns the (ntotal x P) slice, so only the occupied slices are read."""
+    d = rpa_weights(energies, homo, rpamin, rpamax, freq, kind, eta)
+    n_occ = homo + 1 - rpamin
+    n_unocc = rpamax - homo
+    cols = np.asarray(cols, dtype=np.int64)
+    out = None
+    for v in range(n_occ):
+        Mv = np.asarray(M[v])[-n_unocc:, :]
+        blk = Mv.T @ (d[v][:, None] * Mv[:, cols])
+        out = blk if out is None else out + blk
+    out = out.astype(np.result_type(out.dtype, np.float64))
+    out[cols, np.arange(len(cols))] += 1.0
+    return out
+
+
 class BSEOperatorOracle:
     """BSE_OPERATOR<cqp,cx,cd,cd2> -- xtp/src/libxtp/gwbse/bse_operator.cc:28-188,
     xtp/include/votca/xtp/bse_operator.h:40-87, with the OpenMP_CUDA CPU branches
@@ -201,6 +221,43 @@ class BSEOperatorOracle:
                         out[v2 * ct:(v2 + 1) * ct] += B.T @ X[v1 * ct:(v1 + 1) * ct]
         return out
 
+    def matmul_rows(self, X: np.ndarray, rows) -> np.ndarray:
+        """Rows `rows` (BSE indices I(v,c)) of BSE_OPERATOR::matmul: the row construction of
+        bse_operator.cc:61-96 (PrepareMatrix1/2 for Hd / Hd2, Hqp_row) for those rows only, then
+        the Hx block term of bse_operator.cc:98-116 regrouped as cx * V_row (V^T X) with
+        V_{I(v,c),P} = M^P_{vc} (the same sum over v2, c2, P as MultiplyBlocks' v-block pairs).
+        Only the slices the rows touch are read from self.M (any object indexable by slice,
+        returning the (ntotal x P) slice), so full-size operators can be checked row by row."""
+        cqp, cx, cd, cd2 = self.c
+        X = np.asarray(X, dtype=np.float64)
+        vmin = self.vmin - self.rpamin
+        cmin = self.bse_cmin - self.rpamin
+        ct, vt = self.ctotal, self.vtotal
+        ei = self.eps_inv
+        rows = [int(r) for r in rows]
+        out = np.zeros((len(rows), X.shape[1]))
+        for t, r in enumerate(rows):
+            v1, c1 = divmod(r, ct)
+            row = np.zeros(self.size)
+            if cd != 0:
+                T = -cd * self.M[c1 + cmin][cmin:cmin + ct, :] * ei[None, :]
+                row += (T @ self.M[v1 + vmin][vmin:vmin + vt, :].T).reshape(-1, order="F")
+            if cd2 != 0:
+                T = -cd2 * self.M[c1 + cmin][vmin:vmin + vt, :] * ei[None, :]
+                row += (self.M[v1 + vmin][cmin:cmin + ct, :] @ T.T).reshape(-1, order="F")
+            if cqp != 0:
+                row += self.hqp_row(v1, c1)
+            out[t] = row @ X
+        if cx != 0:
+            W = None
+            for v2 in range(vt):
+                blk = self.M[v2 + vmin][cmin:cmin + ct, :].T @ X[v2 * ct:(v2 + 1) * ct]
+                W = blk if W is None else W + blk
+            for t, r in enumerate(rows):
+                v1, c1 = divmod(r, ct)
+                out[t] += cx * (self.M[v1 + vmin][cmin + c1, :] @ W)
+        return out
+
     def diagonal(self) -> np.ndarray:
         """BSE_OPERATOR::diagonal -- bse_operator.cc:134-175."""
         cqp, cx, cd, cd2 = self.c
@@ -223,6 +280,37 @@ class BSEOperatorOracle:
                     e -= cd2 * float(np.sum(self.M[c + cmin][v + vmin] * ei * self.M[v + vmin][c + cmin]))
                 res[self.I(v, c)] = e
         return res
+
+
+def adjust_hqp_size(Hqp: np.ndarray, rpa_energies: np.ndarray, homo: int, rpamin: int, qpmin: int, qpmax: int,
+                    vmin: int, cmax: int) -> np.ndarray:
+    """BSE::AdjustHqpSize (xtp/src/libxtp/gwbse/bse.cc:147-186): the QP Hamiltonian of the GW window
+    [qpmin, qpmax] cut or extended to the BSE window [vmin, cmax]; levels outside the GW window get
+    their RPA input energy on the diagonal (RPA offset vmin - rpamin)."""
+    vt = homo - vmin + 1
+    ct = cmax - homo
+    hs = vt + ct
+    gw = qpmax - qpmin + 1
+    off = vmin - rpamin
+    out = np.zeros((hs, hs))
+    if vmin >= qpmin:
+        start = vmin - qpmin
+        if cmax <= qpmax:
+            out[:, :] = Hqp[start:start + hs, start:start + hs]
+        else:
+            vo = gw - start
+            out[:vo, :vo] = Hqp[start:start + vo, start:start + vo]
+            ve = cmax - qpmax
+            out[np.arange(hs - ve, hs), np.arange(hs - ve, hs)] = rpa_energies[off + vo:off + vo + ve]
+    if vmin < qpmin:
+        oe = qpmin - vmin
+        out[np.arange(oe), np.arange(oe)] = rpa_energies[off:off + oe]
+        out[oe:oe + gw, oe:oe + gw] = Hqp
+        if cmax > qpmax:
+            vo = oe + gw
+            ve = cmax - qpmax
+            out[np.arange(hs - ve, hs), np.arange(hs - ve, hs)] = rpa_energies[off + vo:off + vo + ve]
+    return out
 
 
 def rpa_h2p(M: np.ndarray, energies: np.ndarray, homo: int, rpamin: int, rpamax: int):
@@ -717,13 +805,16 @@ class SigmaCDAOracle:
             einv = np.linalg.inv(rpa_epsilon(M, self.e, homo, rpamin, rpamax, x, "imag")) - np.eye(P)
             self.D.append(-einv + self.k0 * np.exp(-(alpha * x) ** 2))
 
-    def sigma_gq(self, omega, level):
-        """ImaginaryAxisIntegration::SigmaGQDiag (ImaginaryAxisIntegration.cc:140-176)."""
+    def sigma_gq(self, omega, level, rpa_eta=1e-4):
+        """ImaginaryAxisIntegration::SigmaGQDiag (ImaginaryAxisIntegration.cc:140-176). The
+        broadening is the RPA's fixed eta (rpa.h:90), which Sigma_CDA passes in
+        (sigma_cda.cc:125: gq_.SigmaGQDiag(frequency, gw_level, rpa_.getEta())); the GW eta of the
+        options does not enter Sigma_c^CDA."""
         occ = self.homo + 1 - self.rpamin
         Imx = self.M[level + self.qpmin - self.rpamin]
         dE = (omega - self.e).astype(np.complex128)
-        dE[:occ] += 1j * self.eta
-        dE[occ:] -= 1j * self.eta
+        dE[:occ] += 1j * rpa_eta
+        dE[occ:] -= 1j * rpa_eta
         tot = 0.0
         for j, (x, w) in enumerate(zip(self.points, self.weights)):
             if self.symmetry:

@@ -187,6 +187,34 @@ def test_bse_energy_fixtures_gpu(ctx, ints, name, psi, c, offdiag, nroots, full)
 
 
 
+def test_bse_adjust_hqp_fixtures_gpu(ctx, ints):  # test_bse.cc:342-374
+    """BSE::configure's AdjustHqpSize through the library: the cut (vmin=1, cmax=15) and the
+    extension (GW window 1..15 from the cut Hqp, BSE window 0..16) against the reference's
+    fixtures, then the singlet TDA on the cut window against the dense oracle of that window."""
+    S, Saux, V, T = ints
+    C = mmio.read_matrix(os.path.join(G, "bse/MOs.mm"))
+    Hqp = mmio.read_matrix(os.path.join(G, "bse/Hqp.mm"))
+    rpa_e = np.diag(Hqp).copy()
+    t = px.TCMatrix_gwbse(ctx).Initialize(17, 0, 16, 0, 16)
+    t.Fill(T, C, X.pseudo_invsqrt_gwbse(V, Saux))
+    bse = px.BSE(t).configure(px.BSE_options(4, 0, 16, 0, 16, 1, 15), rpa_e, Hqp)
+    ref_cut = mmio.read_matrix(os.path.join(G, "bse/Hqp_cut.mm"))
+    assert mmio.isapprox(bse.getHqp(), ref_cut, 1e-3)
+    assert np.array_equal(bse.getHqp(), X.adjust_hqp_size(Hqp, rpa_e, 4, 0, 0, 16, 1, 15))
+    bse2 = px.BSE(t).configure(px.BSE_options(4, 0, 16, 1, 15, 0, 16), rpa_e, ref_cut)
+    assert mmio.isapprox(bse2.getHqp(), mmio.read_matrix(os.path.join(G, "bse/Hqp_extended.mm")), 1e-3)
+    with pytest.raises(px.XtpGpuError):
+        px.BSE(t).configure(px.BSE_options(4, 0, 16, 0, 16, 1, 17), rpa_e, Hqp)  # cmax past the RPA levels
+    epsinv = bse.SetupDirectInteractionOperator()
+    A = px.SingletOperator_TDA(epsinv, t, bse.getHqp()).configure(bse.operator_options())
+    ds = px.DavidsonSolver()
+    ds.set_tolerance("lapack")
+    ds.set_max_search_space(30)
+    ev, _ = ds.solve(A, 3)
+    dense = X.bse_dense(1, 2, 1, 0, epsinv, t.get_tensor(), bse.getHqp(), 4, 0, 1, 15)
+    assert np.abs(ev - np.linalg.eigvalsh(dense)[:3]).max() < 1e-9
+
+
 def test_full_bse_initial_guess_gpu(ctx, ints):
     """The initial-guess overload (davidsonsolver.h:123-152) on the full BSE of the reference's test:
     started from converged [X; Y] it converges at once to the same roots; a guess with fewer
@@ -742,6 +770,12 @@ def test_sigma_cda_fixture_gpu(ctx, ints):  # test_sigma_cda.cc:41-113
     ora = X.SigmaCDAOracle(M, CDA_E, 4, 0, 16, 0, 1e-3, 1e-3, q.points(), q.weights(), q.use_symmetry())
     for l, w in ((3, -0.2), (7, 0.9), (12, 2.5)):
         assert abs(cda.diag_batch([l], [w])[0] - ora.diag(w, l)) < 1e-9
+    # the GW eta does not enter Sigma_c^CDA (sigma_cda.cc:107,125 pass the RPA eta, rpa.h:90)
+    cda2 = px.Sigma_CDA(t, rpa).configure(4, 0, 16, 0, 16, eta=1e-2, alpha=1e-3, quadrature_scheme="legendre",
+                                          order=100)
+    cda2.PrepareScreening()
+    lv, wv = [3, 7, 12], [-0.2, 0.9, 2.5]
+    assert np.abs(np.asarray(cda2.diag_batch(lv, wv)) - np.asarray(cda.diag_batch(lv, wv))).max() < 1e-12
 
 
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("bf16x3", 2e-5)])

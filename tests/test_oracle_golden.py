@@ -82,6 +82,38 @@ def test_bse_operator_fixtures(bse_op_inputs, name, c):  # test_bse_operator.cc:
     assert np.abs(D - H).max() < 1e-12
 
 
+@pytest.mark.parametrize("c", [(1, 2, 1, 0), (0, 2, 0, 1), (1, 0, 1, 0), (0, 1, 0, 0)])
+def test_bse_operator_rows_match_full(bse_op_inputs, c):
+    """The row-restricted restatement (used at full single-segment size) against the full row
+    loop pinned above, on the reference's CH4 operator inputs (vmin offset and off-diagonal Hqp)
+    and on a random case with the dense closed form."""
+    M, Hqp = bse_op_inputs
+    op = X.BSEOperatorOracle(*c, EPS_INV_BSE_OP, M, Hqp, 4, 0, 0, 1, 10)
+    Xv = np.random.default_rng(5).normal(size=(op.size, 4))
+    rows = [0, 3, op.size // 2, op.size - 1]
+    assert np.abs(op.matmul_rows(Xv, rows) - op.matmul(Xv)[rows]).max() < 1e-12
+    rng = np.random.default_rng(6)
+    Mr = rng.normal(size=(12, 12, 7))
+    Mr = 0.5 * (Mr + Mr.transpose(1, 0, 2))
+    ei = rng.uniform(0.2, 1.0, 7)
+    Hq = np.diag(np.sort(rng.uniform(-1, 1, 12)))
+    op = X.BSEOperatorOracle(*c, ei, Mr, Hq, 4, 0, 0, 1, 10)
+    D = X.bse_dense(*c, ei, Mr, Hq, 4, 0, 1, 10)
+    Xv = rng.normal(size=(op.size, 3))
+    rows = [1, 7, op.size - 2]
+    assert np.abs(op.matmul_rows(Xv, rows) - (D @ Xv)[rows]).max() < 1e-12
+
+
+def test_eps_columns_match_full():
+    rng = np.random.default_rng(8)
+    M = rng.normal(size=(9, 9, 11))
+    e = np.sort(rng.uniform(-1, 1, 9))
+    cols = [0, 4, 10]
+    for kind, f in (("imag", 0.7), ("real", 0.3), ("complex", 0.5 + 0.5j)):
+        full = X.rpa_epsilon(M, e, 2, 0, 8, f, kind)
+        assert np.abs(X.rpa_epsilon_columns(M, e, 2, 0, 8, f, kind, cols) - full[:, cols]).max() < 1e-12
+
+
 def test_eps_brute_force_tiny():
     rng = np.random.default_rng(1)
     M = rng.normal(size=(5, 5, 4))
@@ -232,6 +264,13 @@ def test_sigma_cda_fixture(ints):  # test_sigma_cda.cc:41-113 (Legendre order 10
     cda = X.SigmaCDAOracle(M, CDA_E, 4, 0, 16, 0, 1e-3, 1e-3, q.points(), q.weights(), q.use_symmetry())
     c = np.array([cda.diag(CDA_E[l], l) for l in range(17)])
     assert mmio.isapprox(c, np.diag(mmio.read_matrix(os.path.join(G, "sigma_cda/c_ref.mm"))), 1e-5)
+    # Sigma_c^CDA takes the RPA's fixed eta (rpa.h:90) in both residues and integral
+    # (sigma_cda.cc:107,125): the GW eta of the options must not change it ...
+    cda2 = X.SigmaCDAOracle(M, CDA_E, 4, 0, 16, 0, 1e-2, 1e-3, q.points(), q.weights(), q.use_symmetry())
+    for l, w in ((3, -0.2), (7, 0.9), (12, 2.5)):
+        assert cda2.diag(w, l) == cda.diag(w, l)
+    # ... while the integrand does depend on its broadening, so the check above is not vacuous
+    assert abs(cda.sigma_gq(0.9, 7, rpa_eta=1e-2) - cda.sigma_gq(0.9, 7)) > 1e-7
 
 
 BSE_CASES = [  # test_bse.cc:40-330: (fixture, psi fixture, operator, use Hqp off-diagonals, roots, full BSE)
@@ -282,6 +321,21 @@ def test_bse_energy_fixtures(bse_chain, name, psi, c, offdiag, nroots, full):
     if full:
         RA = unit_columns(mmio.read_matrix(os.path.join(G, "bse/singlets_psi_AR_btda.mm")))
         assert np.abs(np.linalg.norm(RA.T @ unit_columns(Yv), axis=0) - 1).max() < 1e-5
+
+
+def test_adjust_hqp_size_fixtures():  # test_bse.cc:342-374
+    """BSE::AdjustHqpSize cut (vmin=1, cmax=15 of the 0..16 GW window) and extension (GW window
+    1..15 from the cut Hqp, BSE window 0..16, missing levels from the RPA input energies =
+    diag(Hqp)) against the reference's fixtures; both with homo=4, rpamin=0."""
+    Hqp = mmio.read_matrix(os.path.join(G, "bse/Hqp.mm"))
+    rpa_e = np.diag(Hqp).copy()
+    cut = X.adjust_hqp_size(Hqp, rpa_e, 4, 0, 0, 16, 1, 15)
+    ref_cut = mmio.read_matrix(os.path.join(G, "bse/Hqp_cut.mm"))
+    assert mmio.isapprox(cut, ref_cut, 1e-3)
+    ext = X.adjust_hqp_size(ref_cut, rpa_e, 4, 0, 1, 15, 0, 16)
+    assert mmio.isapprox(ext, mmio.read_matrix(os.path.join(G, "bse/Hqp_extended.mm")), 1e-3)
+    # identity when the BSE window is the GW window
+    assert np.array_equal(X.adjust_hqp_size(Hqp, rpa_e, 4, 0, 0, 16, 0, 16), Hqp)
 
 
 def test_ham_davidson_oracle_fixture(bse_chain):

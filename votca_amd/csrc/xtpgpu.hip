@@ -826,8 +826,73 @@ void ensure_zeroed(DevBuf& b, size_t bytes, hipStream_t st) {
   launch_map(st, bytes, [=] __device__(size_t i) { p[i] = 0; });
 }
 
-void bse_matmul_impl(xtpgpu_bseop op, int precision, const double* X, long ldx, long k, double* Y, long ldy) {
+// First-stage scratch budget (bytes) of the aux-chunked Hd / Hd2 contraction: 3 GiB, or
+// XTPGPU_HD_CHUNK_BYTES when set (a small value forces several uneven aux chunks).
+size_t hd_chunk_budget() {
+  const char* e = std::getenv("XTPGPU_HD_CHUNK_BYTES");
+  if (e && *e) {
+    const long long v = std::atoll(e);
+    if (v > 0) return (size_t)v;
+  }
+  return (size_t)3 << 30;
+}
+
+// Consecutive 256-byte aligned slices of a fixed device region: the scratch of one H X call
+// when it must not allocate (the no-alloc entry point hands in the library's static region).
+struct ScratchArena {
+  unsigned char* base = nullptr;
+  size_t cap = 0, used = 0;
+  static size_t up(size_t x) { return (x + 255) & ~(size_t)255; }
+  size_t left() const { return up(used) < cap ? cap - up(used) : 0; }
+  void* take(size_t bytes) {
+    const size_t off = up(used);
+    if (off + bytes > cap)
+      throw Error(XTPGPU_ERR_SHAPE, "H X needs " + std::to_string(off + bytes) + " bytes of scratch, more than the " +
+                                        std::to_string(cap) + "-byte no-alloc region");
+    used = off + bytes;
+    return base + off;
+  }
+};
+
+void zero_bytes(hipStream_t st, void* p, size_t bytes) {
+  unsigned char* q = reinterpret_cast<unsigned char*>(p);
+  launch_map(st, bytes, [=] __device__(size_t i) { q[i] = 0; });
+}
+
+// Scratch bytes of one H X call other than the stage-1 intermediate, and the stage-1 bytes per
+// aux function (both planes), for the given precision and block width.
+struct MatmulPlan {
+  size_t xp = 0, xh = 0, xl = 0, part = 0, wbuf = 0, per_p = 0;
+};
+
+MatmulPlan plan_matmul(const xtpgpu_bseop_s* op, int precision, long k) {
+  MatmulPlan pl;
+  const long vt = op->vt, ct = op->ct, P = op->P, Np = vt * op->ctp;
+  const bool m16 = precision != XTPGPU_FP64, split = precision == XTPGPU_BF16X3;
+  const bool need16 = m16 && (op->cd || op->cd2);
+  pl.xp = (size_t)Np * k * 8;
+  if (need16) {
+    pl.xh = (size_t)Np * k * 2;
+    if (split) pl.xl = (size_t)Np * k * 2;
+  }
+  if (op->cd || op->cd2) pl.per_p = (size_t)std::max(ct, vt) * k * op->vtp * (m16 ? 2 * (split ? 2 : 1) : 8);
+  if (op->cd && m16) pl.part = (size_t)4 * k * ct * vt * 4;  // Hd stage-2 FP32 partials, at most 4 K splits
+  if (op->cx) {
+    const int tiles = (int)(((P + 63) / 64) * ((k + 63) / 64));
+    const GemmDims dd = make_dims<64, 64>((int)P, (int)k, (int)Np, 1, pick_ksplit(tiles, Np, 16), 16, false);
+    pl.part = std::max(pl.part, (size_t)dd.ksplit * P * k * 8);
+    pl.wbuf = (size_t)op->Pp * k * 8;
+  }
+  return pl;
+}
+
+// BSE_OPERATOR::matmul (bse_operator.cc:39-119) on device buffers. Scratch comes from the
+// operator's grow-only buffers, or, with `arena`, from that fixed region only (no allocation,
+// no profiling events; the stage-1 aux chunk shrinks to what the region has left).
+void bse_matmul_impl(xtpgpu_bseop op, int precision, const double* X, long ldx, long k, double* Y, long ldy,
+                     size_t budget = hd_chunk_budget(), ScratchArena* arena = nullptr) {
   xtpgpu_ctx ctx = op->ctx;
+  xtpgpu_ctx pctx = arena ? nullptr : ctx;  // ProfScope is inert without a context
   hipStream_t st = ctx->stream;
   const long N = op->N, vt = op->vt, ct = op->ct, P = op->P;
   const long ctp = op->ctp, vtp = op->vtp, Pp = op->Pp;
@@ -839,20 +904,32 @@ void bse_matmul_impl(xtpgpu_bseop op, int precision, const double* X, long ldx, 
   const bool m16 = precision != XTPGPU_FP64;
   const bool split = precision == XTPGPU_BF16X3;
   const bool need16 = m16 && (op->cd || op->cd2);
+  const MatmulPlan pl = plan_matmul(op, precision, k);
+  auto buf = [&](DevBuf& b, size_t bytes) -> void* {
+    if (bytes == 0) return nullptr;
+    if (arena) return arena->take(bytes);
+    b.ensure(bytes);
+    return b.p;
+  };
+  // grow-only scratch whose never-written pad positions must read as zero: the operator's buffers
+  // are cleared when they grow, an arena slice on every call (it holds the last call's data)
+  auto zbuf = [&](DevBuf& b, size_t bytes) -> void* {
+    if (bytes == 0) return nullptr;
+    if (arena) {
+      void* p = arena->take(bytes);
+      zero_bytes(st, p, bytes);
+      return p;
+    }
+    ensure_zeroed(b, bytes, st);
+    return b.p;
+  };
 
   // padded copies of X (FP64 always; BF16 hi/lo for the mixed contractions)
-  op->xp.ensure((size_t)Np * k * 8);
-  double* Xp = op->xp.as<double>();
-  unsigned short* Xh = nullptr;
-  unsigned short* Xl = nullptr;
-  if (need16) {
-    op->xh.ensure((size_t)Np * k * 2);
-    Xh = op->xh.as<unsigned short>();
-    if (split) {
-      op->xl.ensure((size_t)Np * k * 2);
-      Xl = op->xl.as<unsigned short>();
-    }
-  }
+  double* Xp = static_cast<double*>(buf(op->xp, pl.xp));
+  unsigned short* Xh = need16 ? static_cast<unsigned short*>(buf(op->xh, pl.xh)) : nullptr;
+  unsigned short* Xl = need16 && split ? static_cast<unsigned short*>(buf(op->xl, pl.xl)) : nullptr;
+  void* part_p = buf(op->part, pl.part);
+  double* Wb = op->cx ? static_cast<double*>(zbuf(op->wbuf, pl.wbuf)) : nullptr;
   launch_map(st, (size_t)Np * k, [=] __device__(size_t i) {
     long col = (long)(i / (size_t)ctp), c = (long)(i % (size_t)ctp);  // col = kv*vt + v
     long kv = col / vt, v = col % vt;
@@ -870,7 +947,7 @@ void bse_matmul_impl(xtpgpu_bseop op, int precision, const double* X, long ldx, 
 
   // ---------------- Hqp (bse_operator.cc:121-132): cqp (H_c^T X - X H_v) per vector
   if (op->cqp) {
-    ProfScope prof(ctx, "hqp", 2.0 * N * k * (op->hqp_diag ? 1.0 : (double)(ct + vt)), 8.0 * 3.0 * N * k);
+    ProfScope prof(pctx, "hqp", 2.0 * N * k * (op->hqp_diag ? 1.0 : (double)(ct + vt)), 8.0 * 3.0 * N * k);
     const double cqp = op->cqp;
     if (op->hqp_diag) {
       const double* hc = op->hcd.as<double>();
@@ -892,11 +969,25 @@ void bse_matmul_impl(xtpgpu_bseop op, int precision, const double* X, long ldx, 
     }
   }
 
-  // aux chunk for the two-stage Hd / Hd2 contraction (first-stage scratch <= ~3 GB)
-  const size_t budget = (size_t)3 << 30;
-  const size_t eb1 = m16 ? 2 * (split ? 2 : 1) : 8;
-  const size_t per_p = (size_t)std::max(ct, vt) * k * vtp * eb1;
-  const long pc = std::max<long>(1, std::min<long>(P, (long)(budget / std::max<size_t>(per_p, 1))));
+  // aux chunk for the two-stage Hd / Hd2 contraction (first-stage scratch <= budget, or what the
+  // arena has left); the stage-1 buffers are sized once for the largest chunk
+  long pc = P;
+  unsigned char* s1a = nullptr;  // FP64 intermediate, or the BF16 hi plane
+  unsigned char* s1b = nullptr;  // BF16 lo plane of the split form
+  if (op->cd || op->cd2) {
+    const size_t avail = arena ? (arena->left() > 1024 ? arena->left() - 1024 : 0) : budget;
+    XG_REQUIRE(pl.per_p <= avail, XTPGPU_ERR_SHAPE,
+               "bse matmul: one auxiliary function's stage-1 block (" + std::to_string(pl.per_p) +
+                   " bytes) exceeds the scratch budget");
+    pc = std::max<long>(1, std::min<long>(P, (long)(avail / std::max<size_t>(pl.per_p, 1))));
+    const size_t s1n = (size_t)std::max(ct, vt) * k * pc * vtp;
+    if (!m16) {
+      s1a = static_cast<unsigned char*>(zbuf(op->s1, s1n * 8));
+    } else {
+      s1a = static_cast<unsigned char*>(zbuf(op->s1h, s1n * 2));
+      if (split) s1b = static_cast<unsigned char*>(zbuf(op->s1l, s1n * 2));
+    }
+  }
 
   // ---------------- Hd (bse_operator.cc:70-89, PrepareMatrix1/2): -cd sum_P eps_P M^P_cc' X M^P_vv'^T
   if (op->cd) {
@@ -908,38 +999,34 @@ void bse_matmul_impl(xtpgpu_bseop op, int precision, const double* X, long ldx, 
       const double f1 = 2.0 * M1 * N1 * ct, b1 = eb * ((double)M1 * ct + (double)N * k + (double)M1 * N1);
       const double f2 = 2.0 * ct * vt * np * vt * k, b2 = eb * ((double)M1 * N1 + (double)vt * np * vt) + 16.0 * N * k;
       EpiStage1 e1{nullptr, nullptr, nullptr, nullptr, 0, (int)ct, (int)vt, (int)vtp, (int)k, (int)np, true};
-      const size_t s1n = (size_t)ct * k * np * vtp;
       if (!m16) {
-        ensure_zeroed(op->s1, s1n * 8, st);
-        double* S = op->s1.as<double>();
+        double* S = reinterpret_cast<double*>(s1a);
         e1.S64 = S;
         {
-          ProfScope prof(ctx, "hd_stage1_fp64", f1, b1);
+          ProfScope prof(pctx, "hd_stage1_fp64", f1, b1);
           gemm64<true, true>(st, (int)M1, (int)N1, (int)ctp, 1, 1,
                              Opnd<double>{op->Mcc.as<double>() + p0 * ct * ctp, nullptr, ctp, 0},
                              Opnd<double>{Xp, nullptr, ctp, 0}, nullptr, 0, false, e1);
         }
-        ProfScope prof(ctx, "hd_stage2_fp64", f2, b2);
+        ProfScope prof(pctx, "hd_stage2_fp64", f2, b2);
         gemm64<true, true>(st, (int)ct, (int)vt, (int)(np * vtp), (int)k, 1,
                            Opnd<double>{S, nullptr, k * np * vtp, np * vtp},
                            Opnd<double>{op->Bvv.as<double>() + p0 * vtp, nullptr, P * vtp, 0}, nullptr, 0, false,
                            EpiStore{Y, 1, ct, ldy, alpha, 1.0});
       } else {
-        ensure_zeroed(op->s1h, s1n * 2, st);
-        if (split) ensure_zeroed(op->s1l, s1n * 2, st);
-        unsigned short* Sh = op->s1h.as<unsigned short>();
-        unsigned short* Sl = split ? op->s1l.as<unsigned short>() : nullptr;
+        unsigned short* Sh = reinterpret_cast<unsigned short*>(s1a);
+        unsigned short* Sl = split ? reinterpret_cast<unsigned short*>(s1b) : nullptr;
         e1.Sh = Sh;
         e1.Sl = Sl;
         {
-          ProfScope prof(ctx, split ? "hd_stage1" : "hd_stage1_bf16", f1, b1);
+          ProfScope prof(pctx, split ? "hd_stage1" : "hd_stage1_bf16", f1, b1);
           gemm16<true, true>(st, (int)M1, (int)N1, (int)ctp, 1, 1,
                              Opnd<unsigned short>{op->Mcc_h.as<unsigned short>() + p0 * ct * ctp,
                                                   split ? op->Mcc_l.as<unsigned short>() + p0 * ct * ctp : nullptr,
                                                   ctp, 0},
                              Opnd<unsigned short>{Xh, Xl, ctp, 0}, nullptr, 0, split, false, e1);
         }
-        ProfScope prof(ctx, split ? "hd_stage2" : "hd_stage2_bf16", f2, b2);
+        ProfScope prof(pctx, split ? "hd_stage2" : "hd_stage2_bf16", f2, b2);
         const Opnd<unsigned short> s2a{Sh, Sl, k * np * vtp, np * vtp};
         const Opnd<unsigned short> s2b{op->Bvv_h.as<unsigned short>() + p0 * vtp,
                                        split ? op->Bvv_l.as<unsigned short>() + p0 * vtp : nullptr, P * vtp, 0};
@@ -949,11 +1036,10 @@ void bse_matmul_impl(xtpgpu_bseop op, int precision, const double* X, long ldx, 
         const int ks2 = (int)std::max<long>(1, std::min<long>(4, std::min<long>(512 / std::max<long>(1, tiles2),
                                                                                (np * vtp) / 2048)));
         if (ks2 > 1 && bf16_vec(s2a, s2b, (int)(np * vtp), 0)) {
-          // launched directly so the partial count is exactly dd.ksplit
+          // launched directly so the partial count is exactly dd.ksplit (<= 4, within pl.part)
           const GemmDims dd = make_dims<128, 128>((int)ct, (int)vt, (int)(np * vtp), (int)k, ks2,
                                                   bf16_wide_kstep(split), false);
-          op->part.ensure((size_t)dd.ksplit * k * ct * vt * 4);
-          float* pt = op->part.as<float>();
+          float* pt = static_cast<float*>(part_p);
           XG_HIP((launch_gemm_bf16<128, 128, true, true>(st, dd, s2a, s2b, nullptr, 0, split, EpiPart32{pt, ct, vt, k},
                                                          true)));
           const long nout = k * ct * vt, nsp = dd.ksplit, ctl = ct, vtl = vt;
@@ -974,17 +1060,15 @@ void bse_matmul_impl(xtpgpu_bseop op, int precision, const double* X, long ldx, 
 
   // ---------------- Hd2 (bse_operator.cc:74-86, Hd2 branch): -cd2 sum_P eps_P M^P_{c1 v'} M^P_{v1 c'} X_{v'c'}
   if (op->cd2) {
-    ProfScope prof(ctx, "hd2", 4.0 * P * vt * vt * ct * k, (m16 ? (split ? 4.0 : 2.0) : 8.0) * 2.0 * P * N + 16.0 * N * k);
+    ProfScope prof(pctx, "hd2", 4.0 * P * vt * vt * ct * k, (m16 ? (split ? 4.0 : 2.0) : 8.0) * 2.0 * P * N + 16.0 * N * k);
     const double alpha = -(double)op->cd2;
     for (long p0 = 0; p0 < P; p0 += pc) {
       const long np = std::min(pc, P - p0);
       const long M1 = np * vt, N1 = k * vt;
       EpiStage1 e1{nullptr, nullptr, nullptr, op->eps.as<double>(), p0, (int)vt, (int)vt, (int)vtp, (int)k, (int)np,
                    false};
-      const size_t s1n = (size_t)vt * k * np * vtp;
       if (!m16) {
-        ensure_zeroed(op->s1, s1n * 8, st);
-        double* S = op->s1.as<double>();
+        double* S = reinterpret_cast<double*>(s1a);
         e1.S64 = S;
         gemm64<true, true>(st, (int)M1, (int)N1, (int)ctp, 1, 1,
                            Opnd<double>{op->Mvc.as<double>() + p0 * Np, nullptr, ctp, 0},
@@ -994,10 +1078,8 @@ void bse_matmul_impl(xtpgpu_bseop op, int precision, const double* X, long ldx, 
                            Opnd<double>{S, nullptr, np * vtp, vt * np * vtp}, nullptr, 0, false,
                            EpiStore{Y, 1, ct, ldy, alpha, 1.0});
       } else {
-        ensure_zeroed(op->s1h, s1n * 2, st);
-        if (split) ensure_zeroed(op->s1l, s1n * 2, st);
-        unsigned short* Sh = op->s1h.as<unsigned short>();
-        unsigned short* Sl = split ? op->s1l.as<unsigned short>() : nullptr;
+        unsigned short* Sh = reinterpret_cast<unsigned short*>(s1a);
+        unsigned short* Sl = split ? reinterpret_cast<unsigned short*>(s1b) : nullptr;
         e1.Sh = Sh;
         e1.Sl = Sl;
         gemm16<true, true>(st, (int)M1, (int)N1, (int)ctp, 1, 1,
@@ -1015,14 +1097,11 @@ void bse_matmul_impl(xtpgpu_bseop op, int precision, const double* X, long ldx, 
 
   // ---------------- Hx (bse_operator.cc:98-116, MultiplyBlocks): cx V (V^T X), always FP64
   if (op->cx) {
-    ProfScope prof(ctx, "hx", 4.0 * N * P * k, 8.0 * (2.0 * N * P + 2.0 * N * k));
+    ProfScope prof(pctx, "hx", 4.0 * N * P * k, 8.0 * (2.0 * N * P + 2.0 * N * k));
     const int tiles = (int)(((P + 63) / 64) * ((k + 63) / 64));
     const int ks = pick_ksplit(tiles, Np, 16);
     GemmDims dd = make_dims<64, 64>((int)P, (int)k, (int)Np, 1, ks, 16, false);
-    op->part.ensure((size_t)dd.ksplit * P * k * sizeof(double));
-    ensure_zeroed(op->wbuf, (size_t)Pp * k * sizeof(double), st);
-    double* part = op->part.as<double>();
-    double* Wb = op->wbuf.as<double>();
+    double* part = static_cast<double*>(part_p);
     gemm64<true, true>(st, (int)P, (int)k, (int)Np, 1, ks, Opnd<double>{op->Mvc.as<double>(), nullptr, Np, 0},
                        Opnd<double>{Xp, nullptr, Np, 0}, nullptr, 0, false, EpiPartial{part, (int)P, (int)k, 1});
     // W[kv][Pp] (pads stay zero)
@@ -1040,6 +1119,40 @@ void bse_matmul_impl(xtpgpu_bseop op, int precision, const double* X, long ldx, 
 }
 
 }  // namespace
+
+// BSE::AdjustHqpSize (bse.cc:147-186): host-side, no device work. The BSE window [vmin, cmax]
+// (vtotal = homo - vmin + 1 valence, ctotal = cmax - homo conduction levels) takes the GW-window
+// block of Hqp where the windows overlap; a BSE level outside [qpmin, qpmax] has no QP correction
+// and keeps its RPA input energy (indexed from vmin - rpamin) on the diagonal, zero elsewhere.
+extern "C" int xtpgpu_bse_adjust_hqp(const double* Hqp, int64_t ldh, const double* rpa_energies, int64_t n_rpa,
+                                     int64_t homo, int64_t rpamin, int64_t qpmin, int64_t qpmax, int64_t vmin,
+                                     int64_t cmax, double* out, int64_t ldo) {
+  return guarded([&] {
+    XG_REQUIRE(Hqp && rpa_energies && out, XTPGPU_ERR_ARG, "xtpgpu_bse_adjust_hqp: null argument");
+    const long hs = (homo - vmin + 1) + (cmax - homo);
+    const long gw = qpmax - qpmin + 1;
+    XG_REQUIRE(homo >= vmin && cmax > homo && gw > 0 && qpmin <= homo && qpmax > homo, XTPGPU_ERR_SHAPE,
+               "xtpgpu_bse_adjust_hqp: empty BSE or GW window");
+    XG_REQUIRE(ldh >= gw && ldo >= hs, XTPGPU_ERR_SHAPE, "xtpgpu_bse_adjust_hqp: leading dimension too small");
+    XG_REQUIRE(vmin >= rpamin && cmax - rpamin < n_rpa, XTPGPU_ERR_SHAPE,
+               "xtpgpu_bse_adjust_hqp: BSE window outside the RPA energies");
+    for (long j = 0; j < hs; ++j)
+      for (long i = 0; i < hs; ++i) out[j * ldo + i] = 0.0;
+    // BSE row b is level vmin + b; it lies in the GW window when qpmin <= vmin + b <= qpmax
+    const long shift = vmin - qpmin;  // GW index of BSE row b = b + shift
+    for (long b = 0; b < hs; ++b) {
+      const long gb = b + shift;
+      if (gb < 0 || gb >= gw) {
+        out[b * ldo + b] = rpa_energies[vmin - rpamin + b];
+        continue;
+      }
+      for (long a = 0; a < hs; ++a) {
+        const long ga = a + shift;
+        if (ga >= 0 && ga < gw) out[b * ldo + a] = Hqp[gb * ldh + ga];
+      }
+    }
+  });
+}
 
 extern "C" int xtpgpu_bse_create(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* eps0inv, const double* Hqp,
                                  int64_t ldhqp, int64_t homo, int64_t rpamin, int64_t qpmin, int64_t vmin,

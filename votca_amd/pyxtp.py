@@ -120,6 +120,7 @@ SIGNATURES = {
     "xtpgpu_sigma_ppm_diag": (C.c_int, [_vp, _i64, C.POINTER(_i64), _dp, _dp, _dp]),
     "xtpgpu_sigma_ppm_offdiag": (C.c_int, [_vp, _i64, _dp, _dp]),
     "xtpgpu_gw_solve_qp": (C.c_int, [_vp, _i64, _dp, _dp, C.POINTER(QPOpts), _dp, C.POINTER(C.c_int)]),
+    "xtpgpu_bse_adjust_hqp": (C.c_int, [_dp, _i64, _dp, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _dp, _i64]),
 }
 
 _lib = None
@@ -585,6 +586,53 @@ class DavidsonSolver:
 
     def converged(self):
         return self._converged
+
+
+@dataclass
+class BSE_options:
+    """The window part of BSE::options (bse.h:44-70)."""
+    homo: int
+    rpamin: int
+    rpamax: int
+    qpmin: int
+    qpmax: int
+    vmin: int
+    cmax: int
+
+
+class BSE:
+    """BSE set-up surface (bse.h): configure() stores the options and the BSE-sized QP
+    Hamiltonian (BSE::configure -> AdjustHqpSize, bse.cc:43-59 and 147-186), getHqp() returns it,
+    SetupDirectInteractionOperator() builds the static screening on the device (bse.cc:188-204)."""
+
+    def __init__(self, Mmn: TCMatrix_gwbse):
+        self.Mmn = Mmn
+        self.Hqp = None
+        self.epsilon_0_inv = None
+
+    def configure(self, opt: BSE_options, RPAInputEnergies, Hqp_in):
+        self.opt = opt
+        e = np.ascontiguousarray(RPAInputEnergies, dtype=np.float64)
+        H = _f64(Hqp_in)
+        hs = (opt.homo - opt.vmin + 1) + (opt.cmax - opt.homo)
+        out = np.zeros((hs, hs), order="F")
+        _check(lib().xtpgpu_bse_adjust_hqp(_dptr(H), H.shape[0], _dptr(e), len(e), opt.homo, opt.rpamin, opt.qpmin,
+                                           opt.qpmax, opt.vmin, opt.cmax, _dptr(out), hs))
+        self.RPAInputEnergies = e
+        self.Hqp = out
+        return self
+
+    def getHqp(self) -> np.ndarray:
+        return self.Hqp
+
+    def SetupDirectInteractionOperator(self, energy=0.0) -> np.ndarray:
+        o = self.opt
+        self.epsilon_0_inv = setup_screening(self.Mmn, self.RPAInputEnergies, o.homo, o.rpamin, o.rpamax, energy)
+        return self.epsilon_0_inv
+
+    def operator_options(self) -> "BSEOperator_Options":
+        o = self.opt
+        return BSEOperator_Options(o.homo, o.rpamin, o.qpmin, o.vmin, o.cmax)
 
 
 def setup_screening(Mmn: TCMatrix_gwbse, energies, homo, rpamin, rpamax, energy=0.0) -> np.ndarray:
