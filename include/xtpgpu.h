@@ -152,6 +152,16 @@ int xtpgpu_bse_matmul(xtpgpu_bseop op, int precision, const double* X, int64_t l
 /* Same on device buffers (X, Y: size x k, leading dims ldx/ldy). */
 int xtpgpu_bse_matmul_device(xtpgpu_bseop op, int precision, const double* X, int64_t ldx,
                              int64_t k, double* Y, int64_t ldy);
+/* No counterpart in the reference. xtpgpu_bse_matmul_device for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): all scratch (padded X copies, the aux-chunked Hd / Hd2 intermediate, split-K
+ * partials) comes from a fixed 256 MiB region per GPU that the library holds from the first
+ * xtpgpu_ctx_create on that GPU until the process ends; the aux chunk shrinks to fit it. Calls
+ * whose fixed scratch (X copies, partials and one aux function of the intermediate) exceeds the
+ * region fail with XTPGPU_ERR_SHAPE. Every call on one GPU shares the region and it keeps the
+ * last call's contents: calls that overlap on one GPU (any context) race. */
+int xtpgpu_bse_matmul_device_noalloc(xtpgpu_bseop op, int precision, const double* X, int64_t ldx,
+                                     int64_t k, double* Y, int64_t ldy);
 
 /* ---------------------------------------------------------------- screening, GW */
 typedef struct xtpgpu_sigma_s* xtpgpu_sigma;

@@ -19,6 +19,7 @@
 #include <tuple>
 #include <cmath>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -327,6 +328,37 @@ extern "C" int xtpgpu_device_count(int* count) {
   });
 }
 
+// ------------------------------------------------------------------ no-alloc scratch
+// Device scratch of xtpgpu_bse_matmul_device_noalloc: one copy per GPU, allocated when the
+// library's code object is loaded on that GPU and held until the process ends. In host code the
+// symbol is only a stand-in; kernels get the per-device address that hipGetSymbolAddress returns.
+constexpr size_t kNoAllocScratchBytes = (size_t)256 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<unsigned char*> g_scratch_dev;  // per device id; null until resolved
+
+// Resolve the scratch address on `device` (which must be current) once per device.
+void resolve_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_dev.size() <= device) g_scratch_dev.resize(device + 1, nullptr);
+  if (g_scratch_dev[device]) return;
+  void* p = nullptr;
+  const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (rc != hipSuccess || p == nullptr)
+    throw Error(XTPGPU_ERR_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(rc));
+  g_scratch_dev[device] = static_cast<unsigned char*>(p);
+}
+
+unsigned char* scratch_on(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  XG_REQUIRE(device >= 0 && device < (int)g_scratch_dev.size() && g_scratch_dev[device], XTPGPU_ERR_ARG,
+             "no-alloc scratch not resolved on this device (create a context on it first)");
+  return g_scratch_dev[device];
+}
+}  // namespace
+
 extern "C" int xtpgpu_ctx_create(int device, unsigned flags, xtpgpu_ctx* out) {
   return guarded([&] {
     XG_REQUIRE(out && flags == 0, XTPGPU_ERR_ARG, "xtpgpu_ctx_create: bad arguments");
@@ -334,6 +366,7 @@ extern "C" int xtpgpu_ctx_create(int device, unsigned flags, xtpgpu_ctx* out) {
     XG_HIP(hipGetDeviceCount(&n));
     XG_REQUIRE(device >= 0 && device < n, XTPGPU_ERR_ARG, "xtpgpu_ctx_create: device out of range");
     XG_HIP(hipSetDevice(device));
+    resolve_scratch(device);
     auto c = std::make_unique<xtpgpu_ctx_s>();
     c->device = device;
     XG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -1340,6 +1373,28 @@ extern "C" int xtpgpu_bse_matmul_device(xtpgpu_bseop op, int precision, const do
     XG_REQUIRE(op && X && Y, XTPGPU_ERR_ARG, "null argument");
     XG_HIP(hipSetDevice(op->ctx->device));
     bse_matmul_impl(op, precision, X, ldx, k, Y, ldy);
+    XG_HIP(hipStreamSynchronize(op->ctx->stream));
+  });
+}
+
+extern "C" int xtpgpu_bse_matmul_device_noalloc(xtpgpu_bseop op, int precision, const double* X, int64_t ldx,
+                                                int64_t k, double* Y, int64_t ldy) {
+  return guarded([&] {
+    XG_REQUIRE(op && X && Y, XTPGPU_ERR_ARG, "null argument");
+    XG_REQUIRE(k >= 0 && ldx >= op->N && ldy >= op->N, XTPGPU_ERR_SHAPE,
+               "xtpgpu_bse_matmul_device_noalloc: bad leading dimension");
+    XG_REQUIRE(precision == XTPGPU_FP64 || ((op->mask >> precision) & 1u), XTPGPU_ERR_ARG,
+               "bse matmul: operator was created without this precision");
+    const MatmulPlan pl = plan_matmul(op, precision, k);
+    const size_t fixed = ScratchArena::up(pl.xp) + ScratchArena::up(pl.xh) + ScratchArena::up(pl.xl) +
+                         ScratchArena::up(pl.part) + ScratchArena::up(pl.wbuf) + ScratchArena::up(pl.per_p) + 1024;
+    XG_REQUIRE(fixed <= kNoAllocScratchBytes, XTPGPU_ERR_SHAPE,
+               "xtpgpu_bse_matmul_device_noalloc: operator size x " + std::to_string(k) + " columns needs " +
+                   std::to_string(fixed) + " bytes of scratch, more than its " +
+                   std::to_string(kNoAllocScratchBytes) + "-byte region");
+    XG_HIP(hipSetDevice(op->ctx->device));
+    ScratchArena arena{scratch_on(op->ctx->device), kNoAllocScratchBytes, 0};
+    bse_matmul_impl(op, precision, X, ldx, k, Y, ldy, hd_chunk_budget(), &arena);
     XG_HIP(hipStreamSynchronize(op->ctx->stream));
   });
 }

@@ -120,6 +120,7 @@ SIGNATURES = {
     "xtpgpu_sigma_ppm_diag": (C.c_int, [_vp, _i64, C.POINTER(_i64), _dp, _dp, _dp]),
     "xtpgpu_sigma_ppm_offdiag": (C.c_int, [_vp, _i64, _dp, _dp]),
     "xtpgpu_gw_solve_qp": (C.c_int, [_vp, _i64, _dp, _dp, C.POINTER(QPOpts), _dp, C.POINTER(C.c_int)]),
+    "xtpgpu_bse_matmul_device_noalloc": (C.c_int, [_vp, C.c_int, _vp, _i64, _i64, _vp, _i64]),
     "xtpgpu_bse_adjust_hqp": (C.c_int, [_dp, _i64, _dp, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _dp, _i64]),
 }
 
