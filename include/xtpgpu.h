@@ -99,6 +99,10 @@ int xtpgpu_mmn_fill_ao2mo(xtpgpu_mmn mmn, const double* ao3c, int64_t nbf, const
  * host restatement reproduces it bit for bit); symmetric in (m,n) as a three-centre quantity is. */
 int xtpgpu_mmn_fill_synthetic(xtpgpu_mmn mmn, uint64_t seed, double sigma0, double decay);
 int xtpgpu_mmn_sizes(xtpgpu_mmn mmn, int64_t* mtotal, int64_t* ntotal, int64_t* auxsize);
+/* dst <- src for two tensors of one shape on one device: restores a kept copy where the reference
+ * calls TCMatrix_gwbse::Rebuild (threecenter.h:97-99, refilling from the DFT orbitals) between
+ * evGW iterations. */
+int xtpgpu_mmn_copy(xtpgpu_mmn dst, xtpgpu_mmn src);
 /* Write a transition block: Mmn_[v](c, P) = m[(P*vtotal + v)*ctotal + c] for slices v in
  * [vmin, vmin+vtotal) and rows c in [cmin, cmin+ctotal) (indices relative to mmin / nmin), and
  * with symmetric != 0 also Mmn_[c](v, P), as a three-centre tensor is symmetric in (m, n).
@@ -214,32 +218,58 @@ int xtpgpu_sigma_cda_create(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* rpa_en
                             double alpha, const double* points, const double* weights, int64_t order,
                             int symmetry, xtpgpu_cda* out);
 int xtpgpu_sigma_cda_destroy(xtpgpu_cda c);
+/* The reference's Sigma_CDA reads the RPA input energies by reference (ImaginaryAxisIntegration.h:69,
+ * sigma_cda.cc:85); after RPA::UpdateRPAInputEnergies the caller passes the new ones here
+ * (rpamax - rpamin + 1 values). The dielectric matrices of the last PrepareScreening are kept. */
+int xtpgpu_sigma_cda_set_rpa_energies(xtpgpu_cda c, const double* rpa_energies, int64_t n);
 /* CalcCorrelationDiagElement for n (gw_level, omega) pairs (residues + tail + imaginary-axis
  * integral); dsigma (nullable) by the reference's central difference, h = 1e-3 (sigma_cda.h:56-62). */
 int xtpgpu_sigma_cda_diag(xtpgpu_cda c, int64_t n, const int64_t* gw_levels, const double* omegas,
                           double* sigma, double* dsigma);
 
+/* GW::options QP-solver part (gw.h:55-100) after qp_solver::NormalizeGridSearchOptions
+ * (qp_solver_utils.h:115-170). */
 typedef struct {
   int solver;                    /* 0 = fixedpoint (then grid, then linearisation); 1 = grid (then linearisation) */
   double qp_solver_alpha;        /* Newton damping, gw.h:70 (0.75) */
   double g_sc_limit;             /* QP tolerance (1e-5) */
   int g_sc_max_iterations;       /* Newton iterations (50) */
-  double full_window_half_width; /* grid half width, Ha (0.75; legacy steps*spacing/2) */
-  double dense_spacing;          /* grid spacing, Ha (0.002) */
-  int restrict_search;           /* occupied roots < 0, virtual roots > virtual_min_energy */
+  double full_window_half_width; /* qp_full_window_half_width, Ha (0.75; legacy 0.5 * spacing * (steps - 1)) */
+  double dense_spacing;          /* qp_dense_spacing, Ha (0.002; legacy qp_grid_spacing) */
+  int restrict_search;           /* qp_restrict_search: occupied roots < -zero_margin, virtual > virtual_min_energy */
   double zero_margin;            /* 1e-6 */
   double virtual_min_energy;     /* -0.1 */
   double min_accepted_Z;         /* 0.05 */
   double max_accepted_Z;         /* 1.5 */
-  int bisection_max_iter;        /* 200 */
+  int brent_max_iter;            /* Brent iteration cap; <= 0: g_sc_max_iterations, as the reference passes (gw.cc:444) */
+  int grid_mode;                 /* qp_grid_search_mode: 0 adaptive_with_dense_fallback (default), 1 adaptive, 2 dense */
+  int root_finder;               /* qp_root_finder: 0 bisection (default), 1 brent */
+  double adaptive_shell_width;   /* qp_adaptive_shell_width, Ha (0.025; legacy full width / (max(21, steps/4) - 1)) */
+  int adaptive_shell_count;      /* qp_adaptive_shell_count (0: use the width) */
+  int gw_sc_iteration;           /* evGW iteration (the adaptive scan centres on the linearised root only at 0) */
 } xtpgpu_qp_opts;
 
-/* GW::SolveQP (gw.cc:323-410): solve w = intercept_l + Sigma_c(l, w) for every QP level,
- * evaluating Sigma_c for all unsettled levels in one device pass per step. out[qptotal] = QP
- * energies; converged[l] = 1 where the fixed point or grid search succeeded. */
+/* GW::SolveQP (gw.cc:323-410) with the grid search of gw.cc:433-739 and qp_solver_utils.h:154-712:
+ * solve w = intercept_l + Sigma_c(l, w) for every QP level; the levels run concurrently and every
+ * Sigma_c request round is one device pass over all of them. out[qptotal] = QP energies;
+ * converged[l] = 1 where the fixed point or grid search succeeded (else the linearised value). */
 int xtpgpu_gw_solve_qp(xtpgpu_sigma s, int64_t qptotal, const double* intercepts,
                        const double* frequencies, const xtpgpu_qp_opts* opts, double* out,
                        int* converged);
+/* The same on a Sigma_CDA evaluator (the reference's SolveQP works on any Sigma_base,
+ * sigmafactory.cc:32-36); the derivative is Sigma_CDA's central difference, h = 1e-3. */
+int xtpgpu_gw_solve_qp_cda(xtpgpu_cda c, int64_t qptotal, const double* intercepts,
+                           const double* frequencies, const xtpgpu_qp_opts* opts, double* out,
+                           int* converged);
+/* The same on a caller's Sigma_c evaluator: fn(user, n, levels, omegas, sigma, dsigma) fills
+ * sigma[k] = Sigma_c(levels[k], omegas[k]) and, when dsigma is not null, its derivative; returns
+ * 0 on success. It is called from library threads, one call at a time. homo / qpmin place the
+ * levels for the restricted search (level l is occupied when l + qpmin <= homo). */
+typedef int (*xtpgpu_sigma_fn)(void* user, int64_t n, const int64_t* levels, const double* omegas,
+                               double* sigma, double* dsigma);
+int xtpgpu_gw_solve_qp_fn(xtpgpu_sigma_fn fn, void* user, int64_t qptotal, int64_t homo, int64_t qpmin,
+                          const double* intercepts, const double* frequencies, const xtpgpu_qp_opts* opts,
+                          double* out, int* converged);
 
 /* ---------------------------------------------------------------- Davidson */
 typedef struct {

@@ -12,6 +12,7 @@
 #ifndef XTPGPU_HPP
 #define XTPGPU_HPP
 
+#include <algorithm>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -232,6 +233,41 @@ class DavidsonSolver {
   int64_t n_ = 0;
   std::vector<double> eval_, evec_, evec2_;
 };
+
+// GW::options QP-solver fields with the reference's defaults (gw.h:55-100): grid solver, adaptive scan
+// with dense fallback, bisection, restricted search, 0.75 Ha half window, 0.002 Ha dense spacing,
+// 0.025 Ha shells.
+inline xtpgpu_qp_opts qp_defaults() {
+  xtpgpu_qp_opts o{};
+  o.solver = 1;
+  o.qp_solver_alpha = 0.75;
+  o.g_sc_limit = 1e-5;
+  o.g_sc_max_iterations = 50;
+  o.full_window_half_width = 0.75;
+  o.dense_spacing = 0.002;
+  o.restrict_search = 1;
+  o.zero_margin = 1e-6;
+  o.virtual_min_energy = -0.1;
+  o.min_accepted_Z = 0.05;
+  o.max_accepted_Z = 1.5;
+  o.brent_max_iter = 0;
+  o.grid_mode = 0;
+  o.root_finder = 0;
+  o.adaptive_shell_width = 0.025;
+  o.adaptive_shell_count = 0;
+  o.gw_sc_iteration = 0;
+  return o;
+}
+// the legacy qp_grid_steps / qp_grid_spacing pair, as qp_solver::NormalizeGridSearchOptions maps
+// it (qp_solver_utils.h:115-170)
+inline void qp_legacy_grid(xtpgpu_qp_opts& o, int steps, double spacing) {
+  if (steps <= 1 || spacing <= 0.0) throw std::runtime_error("qp_legacy_grid: need steps > 1 and spacing > 0");
+  const double full = spacing * (double)(steps - 1);
+  const int coarse = std::max(21, steps / 4);
+  o.full_window_half_width = 0.5 * full;
+  o.dense_spacing = spacing;
+  o.adaptive_shell_width = full / (double)(coarse - 1);
+}
 
 // Sigma_PPM (xtp/src/libxtp/self_energy_evaluators/sigma_ppm.h) with the exchange of Sigma_base
 // (sigma_base.h:33-106) and GW::SolveQP (gw.cc:323-410) on the device. PrepareScreening rotates

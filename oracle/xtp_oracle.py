@@ -865,6 +865,52 @@ class SigmaCDAOracle:
         return self.residues(omega, level) + self.sigma_gq(omega, level)
 
 
+def evgw_ppm(M, dft, vxc, homo, rpamin, rpamax, qpmin, qpmax, eta, qp_opts, max_iter, sc_limit, mix_order=0,
+             mix_alpha=0.7, ScaHFX=0.0):
+    """GW::CalculateGWPerturbation with Sigma_PPM (gw.cc:218-310): Sigma_x once; per iteration the
+    PPM screening from the current RPA input energies (ppm.cc:31-59) with the tensor rotated
+    cumulatively (sigma_ppm.cc:32-35), SolveQP at gw_sc_iteration i, Anderson mixing
+    (anderson_mixing.cc) and RPA::UpdateRPAInputEnergies (rpa.cc:32-73) until the RPA energies
+    change by at most sc_limit; then Sigma_c at the final frequencies with the last PPM parameters
+    and the RPA energies as they stand (Sigma_PPM reads them live, sigma_ppm.cc:52). The
+    three-centre rebuild (reset_3c) is left out: every quantity is invariant under the orthogonal
+    aux rotations it undoes. Returns (GWA energies, final RPA energies, iterations)."""
+    from oracle import qp_solver as Q
+    dft = np.asarray(dft, dtype=np.float64)
+    sx = (1.0 - ScaHFX) * sigma_x(M, homo, rpamin, qpmin, qpmax)
+    e_rpa = dft[rpamin:rpamax + 1].copy()
+    freqs = dft[qpmin:qpmax + 1].copy()
+    icpt = freqs + np.diag(sx) - np.diag(vxc)
+    mix = Q.Anderson(mix_order, mix_alpha)
+    Mr = M
+    it = 0
+    w = f = None
+    for i in range(max_iter):
+        it = i + 1
+        phi, w, f = ppm_construct(Mr, e_rpa, homo, rpamin, rpamax)
+        Mr = multiply_right(Mr, phi)
+        if mix_order > 0 and i > 0:
+            mix.update_input(freqs)
+        e_now = e_rpa
+
+        def sig(l, om, d=0):
+            return sigma_ppm_diag(Mr, e_now, w, f, homo, rpamin, qpmin, eta, l, om)[d]
+
+        out, _ = Q.solve_qp(sig, lambda l, om: sig(l, om, 1), icpt, freqs, homo, qpmin, qp_opts, gw_sc_iteration=i)
+        freqs = np.array(out)
+        if max_iter > 1:
+            old = e_rpa.copy()
+            if mix_order > 0 and i > 0:
+                mix.update_output(freqs)
+                freqs = mix.mix()
+            e_rpa = rpa_update_energies(dft, freqs, qpmin, homo, rpamin, rpamax)
+            if np.max(np.abs(e_rpa - old)) <= sc_limit:
+                break
+    sc = np.array([sigma_ppm_diag(Mr, e_rpa, w, f, homo, rpamin, qpmin, eta, l, freqs[l])[0]
+                   for l in range(qpmax - qpmin + 1)])
+    return np.diag(sx) + sc - np.diag(vxc) + dft[qpmin:qpmax + 1], e_rpa, it
+
+
 def screening_at(M, rpa_energies, homo, rpamin, rpamax, omega):
     """BSE::SetupDirectInteractionOperator(energies, omega) (bse.cc:188-204): eps_r(omega) of the
     RPA energies, M rotated to its eigenvectors, eps^-1 = 1/lambda (lambda > 1e-8, else 0)."""

@@ -239,6 +239,86 @@ def test_gw_fixture_grid(ints):  # test_gw.cc:111-190 (G0W0-PPM, grid QP solver,
     assert mmio.isapprox(H, ref, 1e-4)
 
 
+def _gw_case(ints, mos, vxcf):
+    M = mmn_for(ints, f"gw/{mos}", 0, 16, 0, 16)
+    e = GW_E
+    sx = X.sigma_x(M, 4, 0, 0, 16)
+    phi, w, f = X.ppm_construct(M, e, 4, 0, 16)
+    Mr = X.multiply_right(M, phi)
+    vxc = mmio.read_matrix(os.path.join(G, f"gw/{vxcf}"))
+    sig = lambda l, om: X.sigma_ppm_diag(Mr, e, w, f, 4, 0, 0, 1e-3, l, om)[0]  # noqa: E731
+    dsig = lambda l, om: X.sigma_ppm_diag(Mr, e, w, f, 4, 0, 0, 1e-3, l, om)[1]  # noqa: E731
+    return Mr, e, w, f, sx, vxc, sig, dsig
+
+
+@pytest.mark.parametrize("mode", ["adaptive_with_dense_fallback", "adaptive", "dense"])
+@pytest.mark.parametrize("case", [("mo_eigenvectors.mm", "vxc.mm", "ref.mm"), ("mo_eigenvectors2.mm", "vxc2.mm", "ref2.mm")])
+def test_qp_solver_gw_fixtures(ints, case, mode):  # test_gw.cc:111-304
+    """The restated GW::SolveQP (grid solver, legacy 601 x 0.005 Ha window normalised as
+    NormalizeGridSearchOptions does) reproduces the G0W0-PPM QP energies and H_QP of gw/ref.mm and
+    gw/ref2.mm (isApprox 1e-4) in every grid mode; Brent agrees with bisection to 1e-5
+    (test_gw.cc:306-340); the canonical controls of test_gw.cc:262-304 reproduce ref2."""
+    from oracle import qp_solver as Q
+    Mr, e, w, f, sx, vxc, sig, dsig = _gw_case(ints, case[0], case[1])
+    icpt = e + np.diag(sx) - np.diag(vxc)
+    ref = mmio.read_matrix(os.path.join(G, f"gw/{case[2]}"))
+    res = {}
+    for rf in ("bisection", "brent"):
+        o = Q.from_legacy_grid(601, 0.005, qp_grid_search_mode=mode, qp_root_finder=rf)
+        out, conv = Q.solve_qp(sig, dsig, icpt, e, 4, 0, o)
+        assert all(conv)
+        scd = np.array([sig(l, out[l]) for l in range(17)])
+        res[rf] = np.diag(sx) + scd - np.diag(vxc) + e
+        assert mmio.isapprox(res[rf], np.diag(ref), 1e-4), rf
+    assert mmio.isapprox(res["bisection"], res["brent"], 1e-5)
+    gwa = res["bisection"]
+    scd = np.array([sig(l, out[l]) for l in range(17)])
+    H = sx + np.diag(scd) - vxc + np.diag(e)
+    for l1 in range(17):
+        for l2 in range(17):
+            if l1 != l2:
+                H[l1, l2] += X.sigma_ppm_offdiag(Mr, e, w, f, 4, 0, 0, 1e-3, l1, l2, gwa[l1], gwa[l2])
+    assert mmio.isapprox(H, ref, 1e-4)
+    if case[2] == "ref2.mm" and mode == "adaptive_with_dense_fallback":
+        o = Q.QPOptions(qp_full_window_half_width=1.5, qp_dense_spacing=0.005, qp_adaptive_shell_width=0.02)
+        out, _ = Q.solve_qp(sig, dsig, icpt, e, 4, 0, o)
+        scd = np.array([sig(l, out[l]) for l in range(17)])
+        assert mmio.isapprox(np.diag(sx) + scd - np.diag(vxc) + e, np.diag(ref), 1e-4)
+
+
+def test_qp_solver_closed_forms():
+    """Single-pole Sigma = a/(w - b): the roots of w = c + a/(w - b) are the roots of a quadratic.
+    Fixed point, bisection and Brent land on the root within g_sc_limit; the root nearer the pole
+    has Z = 1/(1 + a/(w-b)^2) < 0.05 and is rejected by the acceptance test, so the grid picks
+    the other one; the reference's linearisation step divides by 2 - dSigma/dw."""
+    from oracle import qp_solver as Q
+    a, b, c = 0.0004, 0.5013, 0.25
+    sig = lambda l, w: a / (w - b)  # noqa: E731
+    dsig = lambda l, w: -a / (w - b) ** 2  # noqa: E731
+    roots = np.sort(np.roots([1.0, -(c + b), c * b - a]).real)
+    Zs = 1.0 / (1.0 + a / (roots - b) ** 2)
+    for rf in ("bisection", "brent"):
+        for mode in ("adaptive", "dense", "adaptive_with_dense_fallback"):
+            o = Q.QPOptions(qp_root_finder=rf, qp_grid_search_mode=mode, qp_restrict_search=False, g_sc_limit=1e-9)
+            out, conv = Q.solve_qp(sig, dsig, [c], [0.2], 0, 0, o)
+            assert conv[0]
+            best = roots[np.argmax(Zs - 0.1 * np.abs(roots - 0.2))]
+            assert abs(out[0] - best) < 1e-8, (rf, mode, out[0], roots)
+    o = Q.QPOptions(qp_solver="fixedpoint", g_sc_limit=1e-10, qp_solver_alpha=1.0)
+    out, conv = Q.solve_qp(sig, dsig, [c], [0.2], 0, 0, o)
+    assert conv[0] and np.min(np.abs(roots - out[0])) < 1e-8
+    f = Q.QPFunc(0, sig, dsig, c)
+    assert abs(Q.linearisation(f, 0.2) - (0.2 + (c - 0.2 + sig(0, 0.2)) / (2.0 - dsig(0, 0.2)))) < 1e-15
+    # a narrow window around the near-pole root only: rejected (Z < 0.05), so no accepted root
+    near = roots[np.argmin(Zs)]
+    assert Zs.min() < 0.05
+    o = Q.QPOptions(qp_grid_search_mode="dense", qp_restrict_search=False, qp_full_window_half_width=0.004,
+                    qp_dense_spacing=0.0005, g_sc_limit=1e-9)
+    f = Q.QPFunc(0, sig, dsig, c)
+    assert Q.windowed_dense(f, near, near - 0.004, near + 0.004, False, o) is None
+    assert abs(Q.windowed_dense(f, near, near - 0.004, near + 0.004, True, o) - near) < 1e-8
+
+
 def test_qp_linearisation_closed_form():  # gw.cc:412-431 with a single-pole Sigma
     a, b, w0, icpt = 0.02, 0.7, 0.3, 0.25
     sig = lambda w: (a / (w - b), -a / (w - b) ** 2)

@@ -28,6 +28,7 @@
 #include "gemm_pipe.h"
 #include "host_eigen.h"
 #include "kernels_misc.h"
+#include "qp_solver.h"
 #include "xtpgpu_internal.h"
 
 using namespace xg;
@@ -643,6 +644,22 @@ extern "C" int xtpgpu_mmn_fill_synthetic(xtpgpu_mmn mmn, uint64_t seed, double s
     });
     XG_HIP(hipStreamSynchronize(ctx->stream));
     mmn->version++;
+  });
+}
+
+extern "C" int xtpgpu_mmn_copy(xtpgpu_mmn dst, xtpgpu_mmn src) {
+  return guarded([&] {
+    XG_REQUIRE(dst && src, XTPGPU_ERR_ARG, "xtpgpu_mmn_copy: null argument");
+    XG_REQUIRE(dst->mtotal == src->mtotal && dst->ntotal == src->ntotal && dst->aux == src->aux, XTPGPU_ERR_SHAPE,
+               "xtpgpu_mmn_copy: tensors of different shape");
+    XG_REQUIRE(dst->ctx->device == src->ctx->device, XTPGPU_ERR_ARG, "xtpgpu_mmn_copy: tensors on different devices");
+    XG_HIP(hipSetDevice(dst->ctx->device));
+    // order after pending work on both tensors' streams
+    XG_HIP(hipStreamSynchronize(src->ctx->stream));
+    XG_HIP(hipMemcpyAsync(dst->d(), src->d(), src->pstride() * (size_t)src->aux * 8, hipMemcpyDeviceToDevice,
+                          dst->ctx->stream));
+    XG_HIP(hipStreamSynchronize(dst->ctx->stream));
+    dst->version++;
   });
 }
 

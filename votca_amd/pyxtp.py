@@ -52,7 +52,123 @@ class QPOpts(C.Structure):
                 ("g_sc_max_iterations", C.c_int), ("full_window_half_width", C.c_double),
                 ("dense_spacing", C.c_double), ("restrict_search", C.c_int), ("zero_margin", C.c_double),
                 ("virtual_min_energy", C.c_double), ("min_accepted_Z", C.c_double),
-                ("max_accepted_Z", C.c_double), ("bisection_max_iter", C.c_int)]
+                ("max_accepted_Z", C.c_double), ("brent_max_iter", C.c_int), ("grid_mode", C.c_int),
+                ("root_finder", C.c_int), ("adaptive_shell_width", C.c_double), ("adaptive_shell_count", C.c_int),
+                ("gw_sc_iteration", C.c_int)]
+
+
+_GRID_MODES = {"adaptive_with_dense_fallback": 0, "adaptive": 1, "dense": 2}
+_ROOT_FINDERS = {"bisection": 0, "brent": 1}
+
+
+def qp_options(qp_solver="grid", gw_sc_iteration=0, **kw) -> QPOpts:
+    """GW::options' QP-solver fields (gw.h:55-100) with the reference's names and defaults, after
+    qp_solver::NormalizeGridSearchOptions (qp_solver_utils.h:115-170): the legacy pair
+    qp_grid_steps / qp_grid_spacing fills whatever of qp_full_window_half_width (0.5 * spacing *
+    (steps - 1)), qp_dense_spacing (spacing) and qp_adaptive_shell_width (full width /
+    (max(21, steps // 4) - 1)) is left unset; unset and no legacy pair: 0.75, 0.002, 0.025."""
+    steps, spacing = int(kw.get("qp_grid_steps", 0)), float(kw.get("qp_grid_spacing", 0.0))
+    legacy = steps > 1 and spacing > 0.0
+    hw = float(kw.get("qp_full_window_half_width", -1.0))
+    ds = float(kw.get("qp_dense_spacing", -1.0))
+    sw = float(kw.get("qp_adaptive_shell_width", -1.0))
+    sc = int(kw.get("qp_adaptive_shell_count", 0))
+    if hw <= 0.0:
+        hw = 0.5 * spacing * (steps - 1) if legacy else 0.75
+    if ds <= 0.0:
+        ds = spacing if legacy else 0.002
+    if sc <= 0 and sw <= 0.0:
+        if legacy:
+            coarse = max(21, steps // 4)
+            sw = spacing * (steps - 1) / (coarse - 1) if coarse > 1 else 4.0 * spacing
+        else:
+            sw = 0.025
+    if qp_solver not in ("fixedpoint", "grid"):
+        raise ValueError(f"unknown qp_solver '{qp_solver}'")
+    mode = kw.get("qp_grid_search_mode", "adaptive_with_dense_fallback")
+    if mode not in _GRID_MODES:
+        raise ValueError(f"Unknown gw.qp_grid_search_mode '{mode}'")
+    g_it = int(kw.get("g_sc_max_iterations", 50))
+    return QPOpts(0 if qp_solver == "fixedpoint" else 1, float(kw.get("qp_solver_alpha", 0.75)),
+                  float(kw.get("g_sc_limit", 1e-5)), g_it, hw, ds, 1 if kw.get("qp_restrict_search", True) else 0,
+                  float(kw.get("qp_zero_margin", 1e-6)), float(kw.get("qp_virtual_min_energy", -0.1)),
+                  float(kw.get("min_accepted_Z", 0.05)), float(kw.get("max_accepted_Z", 1.5)), g_it,
+                  _GRID_MODES[mode], _ROOT_FINDERS[kw.get("qp_root_finder", "bisection")], sw, sc,
+                  int(gw_sc_iteration))
+
+
+_SIGMA_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                        C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+
+def SolveQP_with(sigma_fn, intercepts, frequencies, homo, qpmin, **opts):
+    """GW::SolveQP in the library on a caller's Sigma_c: sigma_fn(levels, omegas, want_derivative)
+    returns (sigma, dsigma or None) as arrays. Returns (QP energies, converged flags)."""
+    ic = np.ascontiguousarray(intercepts, dtype=np.float64)
+    fr = np.ascontiguousarray(frequencies, dtype=np.float64)
+    n = len(ic)
+    err = []
+
+    def cb(_user, k, lv, om, sg, dsg):
+        try:
+            lvs = np.ctypeslib.as_array(lv, shape=(k,)).copy()
+            oms = np.ctypeslib.as_array(om, shape=(k,)).copy()
+            s, d = sigma_fn(lvs, oms, bool(dsg))
+            np.ctypeslib.as_array(sg, shape=(k,))[:] = s
+            if dsg:
+                np.ctypeslib.as_array(dsg, shape=(k,))[:] = d
+            return 0
+        except Exception as ex:  # reported through the status code
+            err.append(ex)
+            return -5
+
+    fn = _SIGMA_FN(cb)
+    out = np.zeros(n)
+    conv = (C.c_int * max(n, 1))()
+    o = qp_options(**opts)
+    rc = lib().xtpgpu_gw_solve_qp_fn(fn, None, n, homo, qpmin, _dptr(ic), _dptr(fr), C.byref(o), _dptr(out), conv)
+    if err:
+        raise err[0]
+    _check(rc)
+    return out, np.array(list(conv)[:n], dtype=bool)
+
+
+class Anderson:
+    """Anderson mixing of the evGW input/output energies (xtp/src/libxtp/anderson_mixing.cc:27-99):
+    history of order+1 inputs and outputs; MixHistory solves the small least-squares system for the
+    coefficients and returns alpha * mixed output + (1 - alpha) * mixed input (order 1: plain
+    linear mixing)."""
+
+    def __init__(self):
+        self.inp, self.out = [], []
+
+    def Configure(self, order: int, alpha: float):
+        self.order, self.alpha = order + 1, alpha
+
+    def UpdateOutput(self, v):
+        if len(self.out) > self.order - 1:
+            self.out.pop(0)
+        self.out.append(np.array(v, dtype=np.float64))
+
+    def UpdateInput(self, v):
+        if len(self.out) > self.order - 1:  # the reference tests the output history here too
+            self.inp.pop(0)
+        self.inp.append(np.array(v, dtype=np.float64))
+
+    def MixHistory(self):
+        it = len(self.out)
+        used = it - 1
+        om, im = self.out[-1].copy(), self.inp[-1].copy()
+        if it > 1 and self.order > 1:
+            dn = om - im
+            r = [dn - self.out[used - m] + self.inp[used - m] for m in range(1, it)]
+            A = np.array([[ri @ rj for rj in r] for ri in r])
+            c = np.array([ri @ dn for ri in r])
+            coef = np.linalg.lstsq(A, c, rcond=None)[0]
+            for k in range(1, it):
+                om += coef[k - 1] * (self.out[used - k] - self.out[used])
+                im += coef[k - 1] * (self.inp[used - k] - self.inp[used])
+        return self.alpha * om + (1.0 - self.alpha) * im
 
 
 # name -> (restype, argtypes): the full exported surface of include/xtpgpu.h
@@ -120,7 +236,12 @@ SIGNATURES = {
     "xtpgpu_sigma_ppm_diag": (C.c_int, [_vp, _i64, C.POINTER(_i64), _dp, _dp, _dp]),
     "xtpgpu_sigma_ppm_offdiag": (C.c_int, [_vp, _i64, _dp, _dp]),
     "xtpgpu_gw_solve_qp": (C.c_int, [_vp, _i64, _dp, _dp, C.POINTER(QPOpts), _dp, C.POINTER(C.c_int)]),
+    "xtpgpu_gw_solve_qp_cda": (C.c_int, [_vp, _i64, _dp, _dp, C.POINTER(QPOpts), _dp, C.POINTER(C.c_int)]),
+    "xtpgpu_gw_solve_qp_fn": (C.c_int, [C.c_void_p, C.c_void_p, _i64, _i64, _i64, _dp, _dp, C.POINTER(QPOpts), _dp,
+                                        C.POINTER(C.c_int)]),
     "xtpgpu_bse_matmul_device_noalloc": (C.c_int, [_vp, C.c_int, _vp, _i64, _i64, _vp, _i64]),
+    "xtpgpu_mmn_copy": (C.c_int, [_vp, _vp]),
+    "xtpgpu_sigma_cda_set_rpa_energies": (C.c_int, [_vp, _dp, _i64]),
     "xtpgpu_bse_adjust_hqp": (C.c_int, [_dp, _i64, _dp, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _dp, _i64]),
 }
 
@@ -349,6 +470,30 @@ class RPA:
 
     def getRPAInputEnergies(self):
         return self.energies
+
+    def UpdateRPAInputEnergies(self, dftenergies, gwaenergies, qpmin):
+        """RPA::UpdateRPAInputEnergies / ShiftUncorrectedEnergies / getMaxCorrection
+        (rpa.cc:32-73): the QP window takes the GW energies; levels below qpmin are shifted down by
+        the largest |correction| of the occupied QP levels, levels above qpmax up by that of the
+        virtual QP levels. Indices follow the reference (level numbers used as offsets into the
+        RPA window, which starts at rpamin)."""
+        dft = np.asarray(dftenergies, dtype=np.float64)
+        gw = np.asarray(gwaenergies, dtype=np.float64)
+        rpatotal = self.rpamax - self.rpamin + 1
+        e = dft[self.rpamin:self.rpamin + rpatotal].copy()
+        gwsize = len(gw)
+        e[qpmin - self.rpamin:qpmin - self.rpamin + gwsize] = gw
+        lumo, qpmax = self.homo + 1, qpmin + gwsize - 1
+
+        def maxcorr(lo, hi):
+            r = hi - lo + 1
+            return float(np.max(np.abs(e[lo:lo + r] - dft[lo - self.rpamin:lo - self.rpamin + r])))
+
+        occ, virt = maxcorr(qpmin, self.homo), maxcorr(lumo, qpmax)
+        e[:qpmin] -= occ
+        if self.rpamax - qpmax > 0:
+            e[rpatotal - (self.rpamax - qpmax):] += virt
+        self.energies = e
 
     def calculate_epsilon(self, kind: int, freqs, precision="fp64") -> np.ndarray:
         f = np.atleast_1d(np.asarray(freqs))
@@ -685,6 +830,18 @@ def Perturbative_DynamicalScreening(Mmn: TCMatrix_gwbse, rpa_energies, Hqp, opt:
     return out
 
 
+def _solve_qp(entry, h, qptotal, intercepts, frequencies, opts):
+    o = qp_options(**opts)
+    ic = np.ascontiguousarray(intercepts, dtype=np.float64)
+    fr = np.ascontiguousarray(frequencies, dtype=np.float64)
+    if len(ic) != qptotal or len(fr) != qptotal:
+        raise ValueError("SolveQP: intercepts and frequencies must have qptotal entries")
+    out = np.zeros(qptotal)
+    conv = (C.c_int * qptotal)()
+    _check(entry(h, qptotal, _dptr(ic), _dptr(fr), C.byref(o), _dptr(out), conv))
+    return out, np.array(list(conv), dtype=bool)
+
+
 class Sigma_PPM:
     """Sigma_PPM (sigma_ppm.h) + Sigma_base exchange (sigma_base.h:33-106) on the device."""
 
@@ -719,6 +876,11 @@ class Sigma_PPM:
                                              C.byref(h)))
         self.h = h
 
+    def set_rpa_energies(self, e):
+        """Rebind to new RPA input energies, keeping the PPM parameters of the last screening."""
+        self.rpa.setRPAInputEnergies(e)
+        self._bind()
+
     def diag_batch(self, levels, omegas):
         lv = np.ascontiguousarray(levels, dtype=np.int64)
         om = np.ascontiguousarray(omegas, dtype=np.float64)
@@ -742,18 +904,9 @@ class Sigma_PPM:
         _check(lib().xtpgpu_sigma_ppm_offdiag(self.h, self.qptotal, _dptr(f), _dptr(out)))
         return out
 
-    def SolveQP(self, intercepts, frequencies, solver="fixedpoint", **kw):
-        o = QPOpts(0 if solver == "fixedpoint" else 1, kw.get("qp_solver_alpha", 0.75), kw.get("g_sc_limit", 1e-5),
-                   kw.get("g_sc_max_iterations", 50), kw.get("full_window_half_width", 0.75),
-                   kw.get("dense_spacing", 0.002), 1 if kw.get("restrict_search", True) else 0,
-                   kw.get("zero_margin", 1e-6), kw.get("virtual_min_energy", -0.1), kw.get("min_accepted_Z", 0.05),
-                   kw.get("max_accepted_Z", 1.5), kw.get("bisection_max_iter", 200))
-        ic = np.ascontiguousarray(intercepts, dtype=np.float64)
-        fr = np.ascontiguousarray(frequencies, dtype=np.float64)
-        out = np.zeros(self.qptotal)
-        conv = (C.c_int * self.qptotal)()
-        _check(lib().xtpgpu_gw_solve_qp(self.h, self.qptotal, _dptr(ic), _dptr(fr), C.byref(o), _dptr(out), conv))
-        return out, np.array(list(conv), dtype=bool)
+    def SolveQP(self, intercepts, frequencies, **opts):
+        """GW::SolveQP (gw.cc:323-410) with this evaluator; opts as pyxtp.qp_options."""
+        return _solve_qp(lib().xtpgpu_gw_solve_qp, self.h, self.qptotal, intercepts, frequencies, opts)
 
     def close(self):
         if self.h:
@@ -768,30 +921,80 @@ class Sigma_PPM:
 
 
 class GW:
-    """GW::configure / CalculateGWPerturbation / CalculateHQP / getHQP (gw.cc:35-410, 772-776) for
-    G0W0 with the plasmon-pole self-energy, driving the device kernels."""
+    """GW (gw.h, gw.cc:35-410, 772-776): configure / CalculateGWPerturbation / getGWAResults /
+    CalculateHQP / getHQP, with Sigma_c from the plasmon-pole model ("ppm") or contour deformation
+    ("cda"), G0W0 (gw_sc_max_iterations = 1) or eigenvalue-self-consistent evGW (the loop of
+    gw.cc:218-310: screening rebuilt from the updated RPA energies each iteration, Anderson or
+    linear mixing, the three-centre tensor restored every reset_3c iterations as Mmn.Rebuild()
+    does), all evaluations on the device."""
 
     def __init__(self, Mmn: TCMatrix_gwbse, vxc, dft_energies):
         self.Mmn = Mmn
         self.vxc = np.asarray(vxc, dtype=np.float64)
         self.dft = np.asarray(dft_energies, dtype=np.float64)
 
-    def configure(self, homo, qpmin, qpmax, rpamin, rpamax, eta=1e-3, ScaHFX=0.0, qp_solver="grid", **qp):
+    def configure(self, homo, qpmin, qpmax, rpamin, rpamax, eta=1e-3, ScaHFX=0.0, sigma_integration="ppm",
+                  gw_sc_max_iterations=1, gw_sc_limit=1e-5, gw_mixing_order=0, gw_mixing_alpha=0.7, reset_3c=5,
+                  shift=0.0, alpha=1e-3, quadrature_scheme="legendre", quadrature_order=12, **qp):
         self.homo, self.qpmin, self.qpmax, self.rpamin, self.rpamax = homo, qpmin, qpmax, rpamin, rpamax
         self.qptotal = qpmax - qpmin + 1
-        self.ScaHFX, self.qp_solver, self.qp = ScaHFX, qp_solver, qp
+        self.ScaHFX, self.qp = ScaHFX, qp
+        self.gw_sc_max_iterations, self.gw_sc_limit = gw_sc_max_iterations, gw_sc_limit
+        self.mix_order, self.mix_alpha, self.reset_3c, self.shift = gw_mixing_order, gw_mixing_alpha, reset_3c, shift
         self.rpa = RPA(self.Mmn)
         self.rpa.configure(homo, rpamin, rpamax)
-        self.sigma = Sigma_PPM(self.Mmn, self.rpa).configure(homo, qpmin, qpmax, rpamin, rpamax, eta)
+        if sigma_integration == "ppm":
+            self.sigma = Sigma_PPM(self.Mmn, self.rpa).configure(homo, qpmin, qpmax, rpamin, rpamax, eta)
+        elif sigma_integration == "cda":
+            self.sigma = Sigma_CDA(self.Mmn, self.rpa).configure(homo, qpmin, qpmax, rpamin, rpamax, eta, alpha,
+                                                                 quadrature_scheme, quadrature_order)
+        else:
+            raise ValueError(f"sigma_integration '{sigma_integration}' not available")
+        self.sigma_integration = sigma_integration
         return self
+
+    def _rebuild(self):
+        """Mmn.Rebuild(): the tensor as filled from the DFT orbitals (kept on the device)."""
+        _check(lib().xtpgpu_mmn_copy(self.Mmn.h, self._mmn0.h))
 
     def CalculateGWPerturbation(self):
         self.Sigma_x = (1.0 - self.ScaHFX) * self.sigma.CalcExchangeMatrix()
-        self.rpa.setRPAInputEnergies(self.dft[self.rpamin:self.rpamax + 1])
-        freqs = self.dft[self.qpmin:self.qpmax + 1].copy()
-        self.sigma.PrepareScreening()
+        shifted = self.dft.copy()
+        shifted[self.homo + 1:] += self.shift  # ScissorShift_DFTlevel (gw.cc:209-215)
+        self.rpa.setRPAInputEnergies(shifted[self.rpamin:self.rpamax + 1])
+        freqs = shifted[self.qpmin:self.qpmax + 1].copy()
         intercepts = self.dft[self.qpmin:self.qpmax + 1] + np.diag(self.Sigma_x) - np.diag(self.vxc)
-        freqs, self.qp_converged = self.sigma.SolveQP(intercepts, freqs, self.qp_solver, **self.qp)
+        mixing = Anderson()
+        mixing.Configure(self.mix_order, self.mix_alpha)
+        self._mmn0 = None
+        if self.gw_sc_max_iterations > self.reset_3c:
+            self._mmn0 = TCMatrix_gwbse(self.Mmn.ctx).Initialize(self.Mmn.aux, self.Mmn.mmin, self.Mmn.mmax,
+                                                                 self.Mmn.nmin, self.Mmn.nmax)
+            _check(lib().xtpgpu_mmn_copy(self._mmn0.h, self.Mmn.h))
+        self.gw_iterations = 0
+        self.gw_converged = self.gw_sc_max_iterations == 1
+        for i in range(self.gw_sc_max_iterations):
+            self.gw_iterations = i + 1
+            if i % self.reset_3c == 0 and i != 0:
+                self._rebuild()
+            self.sigma.PrepareScreening()
+            if self.mix_order > 0 and i > 0:
+                mixing.UpdateInput(freqs)
+            freqs, self.qp_converged = self.sigma.SolveQP(intercepts, freqs, gw_sc_iteration=i, **self.qp)
+            if self.gw_sc_max_iterations > 1:
+                old = self.rpa.getRPAInputEnergies().copy()
+                if self.mix_order > 0 and i > 0:
+                    mixing.UpdateOutput(freqs)
+                    freqs = mixing.MixHistory()
+                self.rpa.UpdateRPAInputEnergies(self.dft, freqs, self.qpmin)
+                # GW::Converged (gw.cc:759-770): largest change of the RPA input energies
+                if np.max(np.abs(self.rpa.getRPAInputEnergies() - old)) <= self.gw_sc_limit:
+                    self.gw_converged = True
+                    break
+        self._mmn0 = None
+        # the reference's evaluators read the RPA energies live: the final Sigma_c sees the updated
+        # ones with the last screening (sigma_ppm.cc:52, ImaginaryAxisIntegration.h:69)
+        self.sigma.set_rpa_energies(self.rpa.getRPAInputEnergies())
         self.Sigma_c = np.zeros((self.qptotal, self.qptotal))
         self.Sigma_c[np.diag_indices(self.qptotal)] = self.sigma.CalcCorrelationDiag(freqs)
         return self
@@ -800,6 +1003,8 @@ class GW:
         return np.diag(self.Sigma_x) + np.diag(self.Sigma_c) - np.diag(self.vxc) + self.dft[self.qpmin:self.qpmax + 1]
 
     def CalculateHQP(self):
+        if self.sigma_integration != "ppm":
+            raise NotImplementedError("off-diagonal Sigma_c is built for the plasmon-pole model")
         d = np.diag(self.Sigma_c).copy()
         self.Sigma_c = self.sigma.CalcCorrelationOffDiag(self.getGWAResults())
         self.Sigma_c[np.diag_indices(self.qptotal)] = d
@@ -854,6 +1059,13 @@ class Sigma_CDA:
                                              _dptr(w), len(x), 1 if self.quad.use_symmetry() else 0, C.byref(h)))
         self.h = h
 
+    def set_rpa_energies(self, e):
+        """New RPA input energies for the residues, the tail and the integrand's denominators; the
+        dielectric matrices of the last screening are kept."""
+        self.rpa.setRPAInputEnergies(e)
+        ee = np.ascontiguousarray(e, dtype=np.float64)
+        _check(lib().xtpgpu_sigma_cda_set_rpa_energies(self.h, _dptr(ee), len(ee)))
+
     def diag_batch(self, levels, omegas, derivative=False):
         lv = np.ascontiguousarray(levels, dtype=np.int64)
         om = np.ascontiguousarray(omegas, dtype=np.float64)
@@ -865,6 +1077,10 @@ class Sigma_CDA:
 
     def CalcCorrelationDiag(self, frequencies):
         return self.diag_batch(np.arange(self.qptotal), frequencies)
+
+    def SolveQP(self, intercepts, frequencies, **opts):
+        """GW::SolveQP (gw.cc:323-410) with the CDA evaluator; opts as pyxtp.qp_options."""
+        return _solve_qp(lib().xtpgpu_gw_solve_qp_cda, self.h, self.qptotal, intercepts, frequencies, opts)
 
     def close(self):
         if self.h:
