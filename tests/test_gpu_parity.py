@@ -343,7 +343,8 @@ def test_bf16_kernels_agree(ctx, c, shape, monkeypatch):
     Xv = np.random.default_rng(15).normal(size=(op.size, k))
     ref = op.matmul(Xv, "fp64")
     ys, yp = {}, {}
-    for kern in ("staged32", "staged64", "wide", "pipe", "big", "tall", "db64", "db128", "k64occ3", "k32occ3"):
+    for kern in ("staged32", "staged64", "wide", "pipe", "big", "tall", "db64", "db128", "k64occ3", "k32occ3",
+                 "lds256", "lds128"):
         monkeypatch.setenv("XTPGPU_BF16_KERNEL", kern)
         ys[kern] = op.matmul(Xv, "bf16x3")
         assert relerr(ys[kern], ref) < 2e-5, kern

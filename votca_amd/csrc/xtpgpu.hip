@@ -17,6 +17,7 @@
 #include <cstring>
 #include <limits>
 #include <tuple>
+#include <type_traits>
 #include <cmath>
 #include <memory>
 #include <mutex>
@@ -26,6 +27,7 @@
 
 #include "gemm_kernels.h"
 #include "gemm_pipe.h"
+#include "gemm_lds.h"
 #include "host_eigen.h"
 #include "kernels_misc.h"
 #include "qp_solver.h"
@@ -149,14 +151,22 @@ struct EpiStage1 {
   template <typename V>
   __device__ void put(const Row& ro, long c, V v) const {
     const long idx = ro.off + c;
-    const double x = (double)v * ro.f;
-    if (S64) {
-      S64[idx] = x;
-    } else {
-      unsigned short h, l;
-      split_bf16(x, h, l);
+    if constexpr (std::is_same<V, float>::value) {
+      // BF16 kernels: FP32 accumulator; the split residual x - hi is exact in FP32
+      const float x = scale ? v * (float)ro.f : v;
+      const unsigned short h = bf16_bits_rn(x);
       Sh[idx] = h;
-      if (Sl) Sl[idx] = l;
+      if (Sl) Sl[idx] = bf16_bits_rn(x - bf16_bits_to_f(h));
+    } else {
+      const double x = (double)v * ro.f;
+      if (S64) {
+        S64[idx] = x;
+      } else {
+        unsigned short h, l;
+        split_bf16(x, h, l);
+        Sh[idx] = h;
+        if (Sl) Sl[idx] = l;
+      }
     }
   }
   template <typename V>
@@ -216,6 +226,8 @@ inline int bf16_kernel_choice() {
   if (e && std::strcmp(e, "k32occ3") == 0) return 12;
   if (e && std::strcmp(e, "k32occ4") == 0) return 13;
   if (e && std::strcmp(e, "wide") == 0) return 14;
+  if (e && std::strcmp(e, "lds256") == 0) return 15;
+  if (e && std::strcmp(e, "lds128") == 0) return 16;
   return 1;
 }
 
@@ -225,6 +237,25 @@ void gemm16(hipStream_t st, int M, int N, int K, int batch, int ksplit, Opnd<uns
   if (M <= 0 || N <= 0 || batch <= 0) return;
   const int choice = bf16_kernel_choice();
   if constexpr (AK && BK_) {
+    // direct-to-LDS pipeline with 128-row wave tiles (256 x 256 block, 2 stages; 256 x 128, 3 stages)
+    if ((choice == 15 || choice == 16) && !s && lds_ok(A, B, K, split) && !(lower && choice == 16)) {
+      if (choice == 15) {
+        GemmDims d = make_dims<256, 256>(M, N, std::max(K, 0), batch, ksplit, split ? 32 : 64, lower);
+        if (!lower) d.group = tile_group(d);
+        const char* sc = std::getenv("XTPGPU_LDS_SCHED");
+        const int sched = sc ? std::atoi(sc) : 0;
+        if constexpr (std::is_same<Epi, EpiStage1>::value) {
+          if (sched == 1) { XG_HIP((launch_gemm_bf16_lds<256, 256, 2, Epi, 1>(st, d, A, B, split, epi))); return; }
+          if (sched == 2) { XG_HIP((launch_gemm_bf16_lds<256, 256, 2, Epi, 2>(st, d, A, B, split, epi))); return; }
+        }
+        XG_HIP((launch_gemm_bf16_lds<256, 256, 2>(st, d, A, B, split, epi)));
+      } else {
+        GemmDims d = make_dims<256, 128>(M, N, std::max(K, 0), batch, ksplit, split ? 32 : 64, false);
+        d.group = tile_group(d);
+        XG_HIP((launch_gemm_bf16_lds<256, 128, 3>(st, d, A, B, split, epi)));
+      }
+      return;
+    }
     if (choice == 3 && bf16_vec(A, B, K, s_bs) && (long)M * N >= 256L * 256 * 512) {
       GemmDims d = make_dims<256, 256>(M, N, std::max(K, 0), batch, ksplit, 32, lower);
       XG_HIP((launch_gemm_bf16_big<AK, BK_>(st, d, A, B, s, s_bs, split, epi)));
