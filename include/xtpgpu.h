@@ -191,6 +191,12 @@ int xtpgpu_rpa_h2p(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* energies, int64
  * freq from diag((phi^T eps_i phi)^-1); then Mmn <- Mmn * phi in place. */
 int xtpgpu_ppm_prepare(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* energies, int64_t homo,
                        int64_t rpamin, int64_t rpamax, double* ppm_weight, double* ppm_freq);
+/* Same with the two eps builds in `precision` (XTPGPU_FP64 / XTPGPU_BF16X3 / XTPGPU_BF16); the
+ * eigendecomposition, the inversion and the rotation of the tensor stay FP64. No counterpart in
+ * the reference (FP64 only): the mixed-precision GW form. */
+int xtpgpu_ppm_prepare_prec(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* energies, int64_t homo,
+                            int64_t rpamin, int64_t rpamax, int precision, double* ppm_weight,
+                            double* ppm_freq);
 /* Sigma_PPM evaluator bound to a (PPM-rotated) tensor (sigma_ppm.h; options sigma_base.h:47-57). */
 int xtpgpu_sigma_ppm_create(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* rpa_energies,
                             const double* ppm_weight, const double* ppm_freq, int64_t homo,
@@ -217,6 +223,12 @@ int xtpgpu_sigma_cda_create(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* rpa_en
                             int64_t rpamin, int64_t rpamax, int64_t qpmin, int64_t qpmax, double eta,
                             double alpha, const double* points, const double* weights, int64_t order,
                             int symmetry, xtpgpu_cda* out);
+/* Same with every eps build (the screening of PrepareScreening and the residues' eps_r(|delta| + i eta))
+ * in `precision`; the inversions, LU solves and contractions stay FP64. */
+int xtpgpu_sigma_cda_create_prec(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* rpa_energies, int64_t homo,
+                                 int64_t rpamin, int64_t rpamax, int64_t qpmin, int64_t qpmax, double eta,
+                                 double alpha, const double* points, const double* weights, int64_t order,
+                                 int symmetry, int precision, xtpgpu_cda* out);
 int xtpgpu_sigma_cda_destroy(xtpgpu_cda c);
 /* The reference's Sigma_CDA reads the RPA input energies by reference (ImaginaryAxisIntegration.h:69,
  * sigma_cda.cc:85); after RPA::UpdateRPAInputEnergies the caller passes the new ones here

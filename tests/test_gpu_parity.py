@@ -648,6 +648,26 @@ def test_gw_cda_qp_gpu(ctx, ints):  # test_sigma_cda.cc inputs; GW::SolveQP on S
     assert np.abs(ppm.getGWAResults()[2:5] - gw.getGWAResults()[2:5]).max() < 0.01
 
 
+@pytest.mark.parametrize("sigma", ["ppm", "cda"])
+def test_gw_mixed_screening_gpu(ctx, ints, sigma):
+    """Mixed-precision GW: the screening's eps builds in BF16x3 (split bf16 operands, FP32
+    accumulation), inversions / eigensolves / QP equation in FP64. Per-QP-level error against the
+    FP64 run on the CH4 case (grid solver): below 1e-5 Ha on every level, and all levels converge."""
+    kw = dict(qp_solver="grid", qp_grid_steps=601, qp_grid_spacing=0.005, sigma_integration=sigma,
+              quadrature_order=12)
+    vxc = mmio.read_matrix(os.path.join(G, "gw/vxc.mm"))
+    res = {}
+    for prec in ("fp64", "bf16x3"):
+        gw = px.GW(ch4_tensor(ctx, ints, "gw/mo_eigenvectors.mm"), vxc, GW_E).configure(
+            4, 0, 16, 0, 16, eta=1e-3, screening_precision=prec, **kw)
+        gw.CalculateGWPerturbation()
+        assert gw.qp_converged.all(), prec
+        res[prec] = gw.getGWAResults()
+    err = np.abs(res["bf16x3"] - res["fp64"])
+    print(f"[mixed GW {sigma}] max |dE_QP| = {err.max():.2e} Ha", flush=True)
+    assert err.max() < 1e-5, err
+
+
 @pytest.mark.parametrize("mix,max_it,converges", [(0, 6, False), (2, 20, True)])
 def test_evgw_ppm_gpu(ctx, ints, mix, max_it, converges):  # gw.cc:218-310
     """Eigenvalue-self-consistent GW (plasmon-pole) on the CH4 case to a 1e-5 Ha change of the RPA

@@ -241,6 +241,9 @@ SIGNATURES = {
                                         C.POINTER(C.c_int)]),
     "xtpgpu_bse_matmul_device_noalloc": (C.c_int, [_vp, C.c_int, _vp, _i64, _i64, _vp, _i64]),
     "xtpgpu_mmn_copy": (C.c_int, [_vp, _vp]),
+    "xtpgpu_ppm_prepare_prec": (C.c_int, [_vp, _vp, _dp, _i64, _i64, _i64, C.c_int, _dp, _dp]),
+    "xtpgpu_sigma_cda_create_prec": (C.c_int, [_vp, _vp, _dp, _i64, _i64, _i64, _i64, _i64, C.c_double, C.c_double,
+                                               _dp, _dp, _i64, C.c_int, C.c_int, C.POINTER(_vp)]),
     "xtpgpu_sigma_cda_set_rpa_energies": (C.c_int, [_vp, _dp, _i64]),
     "xtpgpu_bse_adjust_hqp": (C.c_int, [_dp, _i64, _dp, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _dp, _i64]),
 }
@@ -848,9 +851,10 @@ class Sigma_PPM:
     def __init__(self, Mmn: TCMatrix_gwbse, rpa: RPA):
         self.Mmn, self.rpa, self.h = Mmn, rpa, None
 
-    def configure(self, homo, qpmin, qpmax, rpamin, rpamax, eta=1e-3):
+    def configure(self, homo, qpmin, qpmax, rpamin, rpamax, eta=1e-3, screening_precision="fp64"):
         self.homo, self.qpmin, self.qpmax, self.rpamin, self.rpamax, self.eta = homo, qpmin, qpmax, rpamin, rpamax, eta
         self.qptotal = qpmax - qpmin + 1
+        self.screening_precision = screening_precision
         return self
 
     def CalcExchangeMatrix(self) -> np.ndarray:
@@ -863,8 +867,9 @@ class Sigma_PPM:
         P = self.Mmn.aux
         self.ppm_weight, self.ppm_freq = np.zeros(P), np.zeros(P)
         e = np.ascontiguousarray(self.rpa.getRPAInputEnergies())
-        _check(lib().xtpgpu_ppm_prepare(self.Mmn.ctx.h, self.Mmn.h, _dptr(e), self.homo, self.rpamin, self.rpamax,
-                                        _dptr(self.ppm_weight), _dptr(self.ppm_freq)))
+        _check(lib().xtpgpu_ppm_prepare_prec(self.Mmn.ctx.h, self.Mmn.h, _dptr(e), self.homo, self.rpamin,
+                                             self.rpamax, _PREC[self.screening_precision], _dptr(self.ppm_weight),
+                                             _dptr(self.ppm_freq)))
         self._bind()
 
     def _bind(self):
@@ -935,7 +940,8 @@ class GW:
 
     def configure(self, homo, qpmin, qpmax, rpamin, rpamax, eta=1e-3, ScaHFX=0.0, sigma_integration="ppm",
                   gw_sc_max_iterations=1, gw_sc_limit=1e-5, gw_mixing_order=0, gw_mixing_alpha=0.7, reset_3c=5,
-                  shift=0.0, alpha=1e-3, quadrature_scheme="legendre", quadrature_order=12, **qp):
+                  shift=0.0, alpha=1e-3, quadrature_scheme="legendre", quadrature_order=12,
+                  screening_precision="fp64", **qp):
         self.homo, self.qpmin, self.qpmax, self.rpamin, self.rpamax = homo, qpmin, qpmax, rpamin, rpamax
         self.qptotal = qpmax - qpmin + 1
         self.ScaHFX, self.qp = ScaHFX, qp
@@ -944,10 +950,12 @@ class GW:
         self.rpa = RPA(self.Mmn)
         self.rpa.configure(homo, rpamin, rpamax)
         if sigma_integration == "ppm":
-            self.sigma = Sigma_PPM(self.Mmn, self.rpa).configure(homo, qpmin, qpmax, rpamin, rpamax, eta)
+            self.sigma = Sigma_PPM(self.Mmn, self.rpa).configure(homo, qpmin, qpmax, rpamin, rpamax, eta,
+                                                                 screening_precision)
         elif sigma_integration == "cda":
             self.sigma = Sigma_CDA(self.Mmn, self.rpa).configure(homo, qpmin, qpmax, rpamin, rpamax, eta, alpha,
-                                                                 quadrature_scheme, quadrature_order)
+                                                                 quadrature_scheme, quadrature_order,
+                                                                 screening_precision)
         else:
             raise ValueError(f"sigma_integration '{sigma_integration}' not available")
         self.sigma_integration = sigma_integration
@@ -1035,8 +1043,9 @@ class Sigma_CDA:
         self.Mmn, self.rpa, self.h = Mmn, rpa, None
 
     def configure(self, homo, qpmin, qpmax, rpamin, rpamax, eta=1e-3, alpha=1e-3, quadrature_scheme="legendre",
-                  order=12):
+                  order=12, screening_precision="fp64"):
         from . import quadrature
+        self.screening_precision = screening_precision
         self.homo, self.qpmin, self.qpmax, self.rpamin, self.rpamax = homo, qpmin, qpmax, rpamin, rpamax
         self.eta, self.alpha, self.qptotal = eta, alpha, qpmax - qpmin + 1
         self.quad = quadrature.create(quadrature_scheme, order)
@@ -1054,9 +1063,10 @@ class Sigma_CDA:
         x = np.ascontiguousarray(self.quad.points())
         w = np.ascontiguousarray(self.quad.weights())
         h = _vp()
-        _check(lib().xtpgpu_sigma_cda_create(self.Mmn.ctx.h, self.Mmn.h, _dptr(e), self.homo, self.rpamin,
-                                             self.rpamax, self.qpmin, self.qpmax, self.eta, self.alpha, _dptr(x),
-                                             _dptr(w), len(x), 1 if self.quad.use_symmetry() else 0, C.byref(h)))
+        _check(lib().xtpgpu_sigma_cda_create_prec(self.Mmn.ctx.h, self.Mmn.h, _dptr(e), self.homo, self.rpamin,
+                                                  self.rpamax, self.qpmin, self.qpmax, self.eta, self.alpha, _dptr(x),
+                                                  _dptr(w), len(x), 1 if self.quad.use_symmetry() else 0,
+                                                  _PREC[self.screening_precision], C.byref(h)))
         self.h = h
 
     def set_rpa_energies(self, e):
