@@ -343,14 +343,19 @@ def run_matvec(args):
             ", ".join(f"{nm} {t['ms']/args.steps:.2f} ms {t['flops']/max(t['ms']/1e3,1e-30)/1e12:.1f} TF"
                       for nm, t in tm.items()))
     if args.copy_calibration:
-        # a plain 1 GiB device-to-device copy kernel (copy16_kernel): the known-size reading for
-        # the FETCH_SIZE / WRITE_SIZE counters (1 GiB read + 1 GiB written)
         del X, Y
-        nb = 1 << 30
-        src, dst = px.DeviceBuffer(ctx, nb), px.DeviceBuffer(ctx, nb)
-        px._check(px.lib().xtpgpu_copy_device(ctx.h, dst.ptr, src.ptr, nb))
-        out["copy_calibration_bytes"] = {"read": nb, "written": nb}
+        _copy_calibration(px, ctx, out)
     print(json.dumps(out))
+
+
+def _copy_calibration(px, ctx, out):
+    """A plain 1 GiB device-to-device copy kernel (copy16_kernel): the known-size reading for the
+    FETCH_SIZE / WRITE_SIZE counters (1 GiB read + 1 GiB written)."""
+    nb = 1 << 30
+    src, dst = px.DeviceBuffer(ctx, nb), px.DeviceBuffer(ctx, nb)
+    px._check(px.lib().xtpgpu_copy_device(ctx.h, dst.ptr, src.ptr, nb))
+    ctx.synchronize()
+    out["copy_calibration_bytes"] = {"read": nb, "written": nb}
 
 
 def run_large_eps(args):
@@ -395,6 +400,9 @@ def run_large_eps(args):
         log(f"[large-eps] {prec}: kernel {res[prec]['kernel_s']:.2f} s ({res[prec]['tflops']:.1f} TFLOP/s), "
             f"wall {wall:.2f} s")
     res["speedup"] = res["fp64"]["kernel_s"] / res[args.mixed]["kernel_s"]
+    if args.copy_calibration:  # the known-size reading for the FETCH_SIZE / WRITE_SIZE counters
+        del out_dev
+        _copy_calibration(px, ctx, res)
     print(json.dumps({"workload": f"large single segment eps(i w), n_occ={o} n_virt={v} n_aux={P}, {nf} frequencies",
                       "algorithmic_flops": F, "sampled_frequencies": sample, **res}))
 

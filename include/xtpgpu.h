@@ -44,6 +44,7 @@ enum { XTPGPU_EPS_IMAG = 0, XTPGPU_EPS_REAL = 1, XTPGPU_EPS_COMPLEX = 2 };
 typedef struct xtpgpu_ctx_s* xtpgpu_ctx;
 typedef struct xtpgpu_mmn_s* xtpgpu_mmn;
 typedef struct xtpgpu_bseop_s* xtpgpu_bseop;
+typedef struct xtpgpu_mfop_s* xtpgpu_mfop;
 
 /* ---------------------------------------------------------------- errors, devices */
 const char* xtpgpu_last_error(void);
@@ -341,6 +342,40 @@ int xtpgpu_davidson_solve_full_guess(xtpgpu_bseop A, xtpgpu_bseop B, int64_t nei
                                      const xtpgpu_davidson_opts* opts, const double* guess, int64_t ldg,
                                      int64_t nguess, double* evals, double* X, double* Y, int64_t ldv,
                                      xtpgpu_davidson_info* info);
+
+/* ---------------------------------------------------------------- matrix-free operators */
+/* MatrixFreeOperator (matrixfreeoperator.h:45-98): what DavidsonSolver consumes, a size, a
+ * diagonal and matmul. The callback writes Y = A X (size x k, column-major, leading dimensions
+ * ldx / ldy) into device memory on `stream` (the context's hipStream_t) for the requested
+ * precision (XTPGPU_FP64, or a mixed form the operator declared in its precision mask) and
+ * returns 0, or a non-zero code that aborts the solve with that code (XTPGPU_ERR_ARG if positive).
+ * X and Y are device buffers owned by the solver, valid for the duration of the call. */
+typedef int (*xtpgpu_matmul_fn)(void* user, int precision, const double* X, int64_t ldx, int64_t k,
+                                double* Y, int64_t ldy, void* stream);
+/* A caller's operator: `diagonal` (host, size values) is copied; precision_mask as for
+ * xtpgpu_bse_create (bit 1<<XTPGPU_FP64 is implied). */
+int xtpgpu_mfop_create(xtpgpu_ctx ctx, int64_t size, const double* diagonal, xtpgpu_matmul_fn fn,
+                       void* user, uint32_t precision_mask, xtpgpu_mfop* out);
+/* The library's BSE operator seen through the same interface (BSE_OPERATOR derives from
+ * MatrixFreeOperator, bse_operator.h:41); the view is valid while `op` lives. */
+int xtpgpu_mfop_from_bse(xtpgpu_bseop op, xtpgpu_mfop* out);
+/* MatrixFreeOperator::matmul on device buffers. */
+int xtpgpu_mfop_matmul_device(xtpgpu_mfop op, int precision, const double* X, int64_t ldx, int64_t k,
+                              double* Y, int64_t ldy);
+int xtpgpu_mfop_destroy(xtpgpu_mfop op);
+/* DavidsonSolver::solve(A, neigen) (SYMM, davidsonsolver.h:64-122) on any matrix-free operator:
+ * the same solver, options, ladder and refinement as xtpgpu_davidson_solve. */
+int xtpgpu_davidson_solve_mf(xtpgpu_mfop op, int64_t neigen, const xtpgpu_davidson_opts* opts,
+                             double* evals, double* evecs, int64_t ldv, xtpgpu_davidson_info* info);
+/* DavidsonSolver::solve on HamiltonianOperator<R, C> = [[R, C], [-C, -R]] (bseoperator_btda.h:44-150)
+ * with the HAM matrix type: guess == null starts from the unit vectors on the lowest positive
+ * diagonal entries (setupInitialEigenvectors, davidsonsolver.cc:180-206; 2*neigen of them), else
+ * from the nguess columns of guess (2*size x nguess, leading dimension ldg). evecs: the unit
+ * 2*size-long eigenvectors, 2*size x neigen, leading dimension ldv >= 2*size (may be null). */
+int xtpgpu_davidson_solve_ham_mf(xtpgpu_mfop R, xtpgpu_mfop C, int64_t neigen,
+                                 const xtpgpu_davidson_opts* opts, const double* guess, int64_t ldg,
+                                 int64_t nguess, double* evals, double* evecs, int64_t ldv,
+                                 xtpgpu_davidson_info* info);
 
 /* ---------------------------------------------------------------- batched segments */
 /* eps(i w) for many segments of uneven size (the per-segment calculate_epsilon_i of

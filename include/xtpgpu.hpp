@@ -1,9 +1,12 @@
 // C++ host surface over the xtpgpu C-ABI, shaped like the reference's operator classes so that
 // VOTCA-XTP's GW loop and DavidsonSolver can use the device path unchanged:
 //   xtpgpu::TCMatrix     ~ TCMatrix_gwbse        (xtp/include/votca/xtp/threecenter.h:70-142)
+//   xtpgpu::MatrixFreeOperator ~ MatrixFreeOperator (xtp/include/votca/xtp/matrixfreeoperator.h:45-98):
+//                          size, diagonal() and a matmul on device buffers; the base the solver takes
+//   xtpgpu::HostMatrixFreeOperator  a MatrixFreeOperator whose matmul runs on the host (the shape of
+//                          the reference's own test operators, test_davidson.cc:127-165)
 //   xtpgpu::BSEOperator  ~ BSE_OPERATOR<...>     (xtp/include/votca/xtp/bse_operator.h:40-87),
-//                          a MatrixFreeOperator (rows(), diagonal(), matmul()) as the solver needs
-//                          (xtp/include/votca/xtp/matrixfreeoperator.h:45-70)
+//                          derived from MatrixFreeOperator as BSE_OPERATOR is
 //   xtpgpu::RPA          ~ RPA                   (xtp/include/votca/xtp/rpa.h:35-115)
 //   xtpgpu::DavidsonSolver ~ DavidsonSolver      (xtp/include/votca/xtp/davidsonsolver.h:47-334)
 //   xtpgpu::Sigma_PPM    ~ Sigma_PPM + GW::SolveQP (sigma_ppm.h, gw.cc:323-410)
@@ -60,26 +63,94 @@ class TCMatrix {
   int64_t ntotal_, mtotal_, aux_;
 };
 
+// MatrixFreeOperator (matrixfreeoperator.h:45-98). A subclass gives diagonal() and matmul_device():
+// Y (size x k, leading dimension ldy) = A X (leading dimension ldx) on device buffers, issued on
+// `stream` (the context's hipStream_t), for XTPGPU_FP64 or a mixed precision it lists in
+// precision_mask(). The library sees it through mfop(): a callback view made on first use.
+class MatrixFreeOperator {
+ public:
+  explicit MatrixFreeOperator(Context& ctx) : ctx_(&ctx) {}
+  virtual ~MatrixFreeOperator() {
+    if (mf_) xtpgpu_mfop_destroy(mf_);
+  }
+  MatrixFreeOperator(const MatrixFreeOperator&) = delete;
+  MatrixFreeOperator& operator=(const MatrixFreeOperator&) = delete;
+  int64_t size() const { return size_; }
+  void set_size(int64_t n) { size_ = n; }
+  int64_t rows() const { return size_; }
+  int64_t cols() const { return size_; }
+  virtual std::vector<double> diagonal() const = 0;
+  virtual void matmul_device(int precision, const double* X, int64_t ldx, int64_t k, double* Y, int64_t ldy,
+                             void* stream) const = 0;
+  virtual unsigned precision_mask() const { return 1u << XTPGPU_FP64; }
+  Context& context() const { return *ctx_; }
+  // the library handle of this operator (valid while the operator lives)
+  virtual xtpgpu_mfop mfop() const {
+    if (!mf_) {
+      const std::vector<double> d = diagonal();
+      if ((int64_t)d.size() != size_) throw std::runtime_error("MatrixFreeOperator: diagonal() has the wrong length");
+      check(xtpgpu_mfop_create(ctx_->get(), size_, d.data(), &MatrixFreeOperator::trampoline,
+                               const_cast<MatrixFreeOperator*>(this), precision_mask(), &mf_));
+    }
+    return mf_;
+  }
+
+ protected:
+  mutable xtpgpu_mfop mf_ = nullptr;
+  int64_t size_ = 0;
+
+ private:
+  static int trampoline(void* user, int precision, const double* X, int64_t ldx, int64_t k, double* Y, int64_t ldy,
+                        void* stream) {
+    try {
+      static_cast<const MatrixFreeOperator*>(user)->matmul_device(precision, X, ldx, k, Y, ldy, stream);
+      return 0;
+    } catch (...) {
+      return XTPGPU_ERR_ARG;
+    }
+  }
+  Context* ctx_;
+};
+
+// A MatrixFreeOperator with a host matmul (column-major size x k in, size x k out), as the
+// reference's test operators are written: the columns travel to the host and back per call.
+class HostMatrixFreeOperator : public MatrixFreeOperator {
+ public:
+  using MatrixFreeOperator::MatrixFreeOperator;
+  virtual void matmul(const double* X, int64_t k, double* Y) const = 0;
+  void matmul_device(int, const double* X, int64_t ldx, int64_t k, double* Y, int64_t ldy, void*) const override {
+    std::vector<double> in((size_t)ldx * k), out((size_t)size_ * k), back((size_t)ldy * k, 0.0);
+    check(xtpgpu_memcpy_d2h(context().get(), in.data(), X, (int64_t)in.size() * 8));
+    std::vector<double> packed((size_t)size_ * k);
+    for (int64_t j = 0; j < k; ++j)
+      std::copy(in.begin() + j * ldx, in.begin() + j * ldx + size_, packed.begin() + j * size_);
+    matmul(packed.data(), k, out.data());
+    for (int64_t j = 0; j < k; ++j) std::copy(out.begin() + j * size_, out.begin() + (j + 1) * size_, back.begin() + j * ldy);
+    check(xtpgpu_memcpy_h2d(context().get(), Y, back.data(), (int64_t)back.size() * 8));
+  }
+};
+
 struct BSEOperatorOptions {  // BSEOperator_Options (bse_operator.h:32-38)
   int64_t homo, rpamin, qpmin, vmin, cmax;
 };
 
 template <int cqp, int cx, int cd, int cd2>
-class BSEOperator {
+class BSEOperator : public MatrixFreeOperator {
  public:
   BSEOperator(Context& ctx, TCMatrix& Mmn, const double* eps0inv, const double* Hqp, int64_t ldhqp,
-              const BSEOperatorOptions& o, unsigned precision_mask = 1u << XTPGPU_FP64) {
+              const BSEOperatorOptions& o, unsigned precision_mask = 1u << XTPGPU_FP64)
+      : MatrixFreeOperator(ctx), mask_(precision_mask | (1u << XTPGPU_FP64)) {
     static_assert(!(cd != 0 && cd2 != 0), "Hamiltonian cannot contain Hd and Hd2 at the same time");
     check(xtpgpu_bse_create(ctx.get(), Mmn.get(), eps0inv, Hqp, ldhqp, o.homo, o.rpamin, o.qpmin, o.vmin, o.cmax, cqp,
                             cx, cd, cd2, precision_mask, &h_));
     check(xtpgpu_bse_size(h_, &size_));
   }
-  ~BSEOperator() { xtpgpu_bse_destroy(h_); }
-  BSEOperator(const BSEOperator&) = delete;
-  BSEOperator& operator=(const BSEOperator&) = delete;
-  int64_t rows() const { return size_; }
-  int64_t cols() const { return size_; }
-  std::vector<double> diagonal() const {
+  ~BSEOperator() override {
+    if (mf_) xtpgpu_mfop_destroy(mf_);
+    mf_ = nullptr;
+    xtpgpu_bse_destroy(h_);
+  }
+  std::vector<double> diagonal() const override {
     std::vector<double> d(size_);
     check(xtpgpu_bse_diagonal(h_, d.data()));
     return d;
@@ -88,11 +159,21 @@ class BSEOperator {
   void matmul(const double* X, int64_t k, double* Y, int precision = XTPGPU_FP64) const {
     check(xtpgpu_bse_matmul(h_, precision, X, size_, k, Y, size_));
   }
+  void matmul_device(int precision, const double* X, int64_t ldx, int64_t k, double* Y, int64_t ldy,
+                     void*) const override {
+    check(xtpgpu_bse_matmul_device(h_, precision, X, ldx, k, Y, ldy));
+  }
+  unsigned precision_mask() const override { return mask_; }
+  // the fused device operator itself, not a callback
+  xtpgpu_mfop mfop() const override {
+    if (!mf_) check(xtpgpu_mfop_from_bse(h_, &mf_));
+    return mf_;
+  }
   xtpgpu_bseop get() const { return h_; }
 
  private:
   xtpgpu_bseop h_ = nullptr;
-  int64_t size_ = 0;
+  unsigned mask_;
 };
 
 // RPA (xtp/include/votca/xtp/rpa.h:35-115): dielectric builds and the two-particle Hamiltonian.
@@ -197,14 +278,23 @@ class DavidsonSolver {
     o_.precision_start = start;
     o_.switch_tolerance = switch_tolerance;
   }
-  // SYMM solve of a BSE operator; returns false when not converged (unconverged roots zeroed,
-  // as storeNotConvergedData does)
-  template <class Op>
-  bool solve(const Op& A, int64_t neigen) {
+  // DavidsonSolver::solve(A, neigen) (SYMM) on any MatrixFreeOperator: a BSEOperator runs its fused
+  // device contraction, other operators their matmul_device. Returns false when not converged
+  // (unconverged roots zeroed, as storeNotConvergedData does).
+  bool solve(const MatrixFreeOperator& A, int64_t neigen) {
     n_ = A.rows();
     eval_.assign(neigen, 0.0);
     evec_.assign(n_ * neigen, 0.0);
-    return finish(xtpgpu_davidson_solve(A.get(), neigen, &o_, eval_.data(), evec_.data(), n_, &info_));
+    return finish(xtpgpu_davidson_solve_mf(A.mfop(), neigen, &o_, eval_.data(), evec_.data(), n_, &info_));
+  }
+  // DavidsonSolver::solve on HamiltonianOperator<R, C> with the HAM matrix type (set_matrix_type("HAM")
+  // in the reference): eigenvalues() and the unit 2*size-long eigenvectors()
+  bool solve_ham(const MatrixFreeOperator& R, const MatrixFreeOperator& C, int64_t neigen) {
+    n_ = 2 * R.rows();
+    eval_.assign(neigen, 0.0);
+    evec_.assign(n_ * neigen, 0.0);
+    return finish(xtpgpu_davidson_solve_ham_mf(R.mfop(), C.mfop(), neigen, &o_, nullptr, 0, 0, eval_.data(),
+                                               evec_.data(), n_, &info_));
   }
   // full BSE through the HamiltonianOperator [[A, B], [-B, -A]] (BSE::Solve_nonhermitian_Davidson)
   template <class OpA, class OpB>

@@ -729,13 +729,13 @@ struct RpaPacked {
   DevBuf A64;          // [P][Kp] FP64
   DevBuf Ah, Al;       // [P][Kp] bf16 hi / lo
   long K = 0;          // transitions n_occ * n_unocc
-  long Kp = 0;         // row length: K rounded up to 8 (16-byte rows for the vector loads), zero pads
+  long Kp = 0;         // row length: K rounded up to 64 (8 below 64) for line-aligned rows, zero pads
 };
 
 void rpa_pack(xtpgpu_mmn mmn, long n_occ, long n_unocc, bool need64, bool need16, RpaPacked& pk) {
   hipStream_t st = mmn->ctx->stream;
   const long P = mmn->aux, K = n_occ * n_unocc, nt = mmn->ntotal, ps = (long)mmn->pstride();
-  const long Kp = (K + 7) / 8 * 8;
+  const long Kp = K >= 64 ? (K + 63) / 64 * 64 : (K + 7) / 8 * 8;  // 128-byte aligned rows
   const long r0 = nt - n_unocc;
   pk.K = K;
   pk.Kp = Kp;
@@ -869,7 +869,9 @@ extern "C" int xtpgpu_rpa_epsilon(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* 
 }
 
 // =================================================================== BSE operator
-// Device layouts (ctp / vtp / Pp = ctotal / vtotal / auxsize rounded up to 8, pads zero):
+// Device layouts (vtp / Pp = vtotal / auxsize rounded up to 8; ctp = ctotal rounded up to 64 when
+// ctotal >= 64, else to 8: rows of ctp elements start on 128-byte lines in BF16 and FP64, so a
+// 64-wide K step reads whole cache lines; pads zero; contractions over c' run to round8(ct)):
 //   X   [(kv, v')][ctp]        the k input vectors, BSE index (v', c')
 //   Mcc [P][c1][ctp]           M^P_{c1 c'}            Hd stage 1 (A, K = c')
 //   Bvv [v1][P][vtp]           eps_P M^P_{v1 v'}      Hd stage 2 (B, K = (P, v'))
@@ -1085,7 +1087,7 @@ void bse_matmul_impl(xtpgpu_bseop op, int precision, const double* X, long ldx, 
         e1.S64 = S;
         {
           ProfScope prof(pctx, "hd_stage1_fp64", f1, b1);
-          gemm64<true, true>(st, (int)M1, (int)N1, (int)ctp, 1, 1,
+          gemm64<true, true>(st, (int)M1, (int)N1, (int)round8(ct), 1, 1,
                              Opnd<double>{op->Mcc.as<double>() + p0 * ct * ctp, nullptr, ctp, 0},
                              Opnd<double>{Xp, nullptr, ctp, 0}, nullptr, 0, false, e1);
         }
@@ -1101,7 +1103,7 @@ void bse_matmul_impl(xtpgpu_bseop op, int precision, const double* X, long ldx, 
         e1.Sl = Sl;
         {
           ProfScope prof(pctx, split ? "hd_stage1" : "hd_stage1_bf16", f1, b1);
-          gemm16<true, true>(st, (int)M1, (int)N1, (int)ctp, 1, 1,
+          gemm16<true, true>(st, (int)M1, (int)N1, (int)round8(ct), 1, 1,
                              Opnd<unsigned short>{op->Mcc_h.as<unsigned short>() + p0 * ct * ctp,
                                                   split ? op->Mcc_l.as<unsigned short>() + p0 * ct * ctp : nullptr,
                                                   ctp, 0},
@@ -1151,7 +1153,7 @@ void bse_matmul_impl(xtpgpu_bseop op, int precision, const double* X, long ldx, 
       if (!m16) {
         double* S = reinterpret_cast<double*>(s1a);
         e1.S64 = S;
-        gemm64<true, true>(st, (int)M1, (int)N1, (int)ctp, 1, 1,
+        gemm64<true, true>(st, (int)M1, (int)N1, (int)round8(ct), 1, 1,
                            Opnd<double>{op->Mvc.as<double>() + p0 * Np, nullptr, ctp, 0},
                            Opnd<double>{Xp, nullptr, ctp, 0}, nullptr, 0, false, e1);
         gemm64<true, true>(st, (int)ct, (int)vt, (int)(np * vtp), (int)k, 1,
@@ -1163,7 +1165,7 @@ void bse_matmul_impl(xtpgpu_bseop op, int precision, const double* X, long ldx, 
         unsigned short* Sl = split ? reinterpret_cast<unsigned short*>(s1b) : nullptr;
         e1.Sh = Sh;
         e1.Sl = Sl;
-        gemm16<true, true>(st, (int)M1, (int)N1, (int)ctp, 1, 1,
+        gemm16<true, true>(st, (int)M1, (int)N1, (int)round8(ct), 1, 1,
                            Opnd<unsigned short>{op->Mvc_h.as<unsigned short>() + p0 * Np,
                                                 split ? op->Mvc_l.as<unsigned short>() + p0 * Np : nullptr, ctp, 0},
                            Opnd<unsigned short>{Xh, Xl, ctp, 0}, nullptr, 0, split, false, e1);
@@ -1258,7 +1260,7 @@ extern "C" int xtpgpu_bse_create(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* e
     XG_REQUIRE(vmin_r >= 0 && cmin_r + ct <= mmn->mtotal && cmin_r + ct <= mmn->ntotal, XTPGPU_ERR_SHAPE,
                "xtpgpu_bse_create: BSE window exceeds the Mmn tensor");
     op->vt = vt; op->ct = ct; op->N = vt * ct; op->P = mmn->aux;
-    op->ctp = round8(ct); op->vtp = round8(vt); op->Pp = round8(op->P);
+    op->ctp = ct >= 64 ? (ct + 63) / 64 * 64 : round8(ct); op->vtp = round8(vt); op->Pp = round8(op->P);
     const long P = op->P, N = op->N, nt = mmn->ntotal, ps = (long)mmn->pstride();
     const long ctp = op->ctp, vtp = op->vtp, Pp = op->Pp, Np = vt * ctp;
     XG_HIP(hipSetDevice(ctx->device));
@@ -1466,6 +1468,7 @@ extern "C" int xtpgpu_bse_matmul(xtpgpu_bseop op, int precision, const double* X
   });
 }
 
+#include "mfop.inc"
 #include "davidson.inc"
 #include "davidson_ham.inc"
 #include "adjoint.inc"

@@ -11,6 +11,7 @@ the C-ABI. Loading fails loudly when the library is missing: there is no CPU fal
 from __future__ import annotations
 
 import ctypes as C
+import inspect
 import os
 from dataclasses import dataclass
 
@@ -22,6 +23,7 @@ LIB_PATH = os.path.join(_HERE, "libxtpgpu.so")
 FP64, BF16, BF16X3 = 0, 1, 3
 EPS_IMAG, EPS_REAL, EPS_COMPLEX = 0, 1, 2
 _PREC = {"fp64": FP64, "bf16": BF16, "bf16x3": BF16X3, FP64: FP64, BF16: BF16, BF16X3: BF16X3}
+_PREC_NAME = {FP64: "fp64", BF16: "bf16", BF16X3: "bf16x3"}
 
 _i64 = C.c_int64
 _dp = C.POINTER(C.c_double)
@@ -213,6 +215,14 @@ SIGNATURES = {
                                              C.POINTER(DavidsonInfo)]),
     "xtpgpu_davidson_solve_full_guess": (C.c_int, [_vp, _vp, _i64, C.POINTER(DavidsonOpts), _dp, _i64, _i64, _dp,
                                                    _dp, _dp, _i64, C.POINTER(DavidsonInfo)]),
+    "xtpgpu_mfop_create": (C.c_int, [_vp, _i64, _dp, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(_vp)]),
+    "xtpgpu_mfop_from_bse": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "xtpgpu_mfop_matmul_device": (C.c_int, [_vp, C.c_int, _vp, _i64, _i64, _vp, _i64]),
+    "xtpgpu_mfop_destroy": (C.c_int, [_vp]),
+    "xtpgpu_davidson_solve_mf": (C.c_int, [_vp, _i64, C.POINTER(DavidsonOpts), _dp, _dp, _i64,
+                                           C.POINTER(DavidsonInfo)]),
+    "xtpgpu_davidson_solve_ham_mf": (C.c_int, [_vp, _vp, _i64, C.POINTER(DavidsonOpts), _dp, _i64, _i64, _dp,
+                                               _dp, _i64, C.POINTER(DavidsonInfo)]),
     "xtpgpu_bse_adjoint": (C.c_int, [_vp, _vp, _i64, _dp, _dp, _i64, _dp, _dp]),
     "xtpgpu_bse_adjoint_device": (C.c_int, [_vp, _vp, _i64, _dp, _dp, _i64, _dp, _vp]),
     "xtpgpu_sigma_cda_create": (C.c_int, [_vp, _vp, _dp, _i64, _i64, _i64, _i64, _i64, C.c_double, C.c_double,
@@ -649,6 +659,125 @@ def Hd2Operator(eps, Mmn, Hqp, **kw):
     return BSE_OPERATOR(0, 0, 0, 1, eps, Mmn, Hqp, **kw)
 
 
+_MATMUL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64,
+                         C.c_void_p)
+
+
+class MatrixFreeOperator:
+    """MatrixFreeOperator (matrixfreeoperator.h:45-98): subclass it and define matmul(X) (numpy,
+    size x k) and diagonal(); set_size(n) before use. The device Davidson calls matmul through a
+    callback: X is copied to the host, matmul runs, Y goes back to the device. An operator that has
+    a device implementation overrides matmul_device(precision, X_ptr, ldx, k, Y_ptr, ldy) instead
+    (device addresses as ints, the library's stream is current).
+
+    `precisions` names the mixed forms matmul accepts besides FP64 (passed as `precision`, one of
+    "fp64", "bf16", "bf16x3", to matmul if it takes that keyword)."""
+
+    precisions = ("fp64",)
+
+    def set_size(self, n: int):
+        self._size = int(n)
+
+    def size(self) -> int:
+        return self._size
+
+    def rows(self) -> int:
+        return self._size
+
+    def cols(self) -> int:
+        return self._size
+
+    def diagonal(self) -> np.ndarray:
+        raise NotImplementedError
+
+    def matmul(self, X: np.ndarray, precision: str = "fp64") -> np.ndarray:
+        raise NotImplementedError
+
+    def __matmul__(self, X):
+        return self.matmul(np.asarray(X, dtype=np.float64))
+
+    def matmul_device(self, precision, X, ldx, k, Y, ldy):
+        n = self._size
+        host = np.empty((ldx, k), order="F")
+        _check(lib().xtpgpu_memcpy_d2h(self._ctx.h, host.ctypes.data_as(_vp), C.c_void_p(X), host.nbytes))
+        if "precision" in inspect.signature(self.matmul).parameters:
+            out = self.matmul(host[:n], precision=_PREC_NAME[precision])
+        else:
+            out = self.matmul(host[:n])
+        out = np.asarray(out, dtype=np.float64)
+        if out.shape != (n, k):
+            raise ValueError(f"matmul returned shape {out.shape}, expected {(n, k)}")
+        Yh = np.zeros((ldy, k), order="F")
+        Yh[:n] = out
+        _check(lib().xtpgpu_memcpy_h2d(self._ctx.h, C.c_void_p(Y), Yh.ctypes.data_as(_vp), Yh.nbytes))
+
+    def _handle(self, ctx):
+        """The library's view of this operator (created once per context)."""
+        if getattr(self, "_h", None) is not None and self._ctx is ctx:
+            return self._h
+        self._ctx, self._err = ctx, []
+
+        def cb(_user, prec, X, ldx, k, Y, ldy, _stream):
+            try:
+                self.matmul_device(int(prec), X, int(ldx), int(k), Y, int(ldy))
+                return 0
+            except Exception as ex:  # reported through the status code
+                self._err.append(ex)
+                return -5
+
+        self._cb = _MATMUL_FN(cb)
+        d = _f64(self.diagonal()).ravel()
+        if d.shape != (self._size,):
+            raise ValueError("diagonal() must return size values")
+        mask = 1
+        for p in self.precisions:
+            mask |= 1 << _PREC[p]
+        h = _vp()
+        _check(lib().xtpgpu_mfop_create(ctx.h, self._size, _dptr(d), C.cast(self._cb, C.c_void_p), None, mask,
+                                        C.byref(h)))
+        self._h = h
+        return h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None:
+            try:
+                lib().xtpgpu_mfop_destroy(h)
+            except Exception:
+                pass
+
+
+class HamiltonianOperator:
+    """HamiltonianOperator<R, C> (bseoperator_btda.h:44-150): [[R, C], [-C, -R]] for the HAM-type
+    Davidson; R and C are MatrixFreeOperator or BSE_OPERATOR instances of equal size."""
+
+    def __init__(self, R, Cop):
+        if _op_size(R) != _op_size(Cop):
+            raise ValueError("R and C operators must have the same size")
+        self.R, self.C = R, Cop
+
+    def rows(self):
+        return 2 * _op_size(self.R)
+
+    def cols(self):
+        return self.rows()
+
+    def matmul(self, X):
+        n = _op_size(self.R)
+        X = np.asarray(X, dtype=np.float64)
+        x, y = X[:n], X[n:]
+        r, c = self.R @ np.hstack([x, y]), self.C @ np.hstack([x, y])
+        k = X.shape[1]
+        return np.vstack([r[:, :k] + c[:, k:], -c[:, :k] - r[:, k:]])
+
+    def __matmul__(self, X):
+        return self.matmul(X)
+
+
+def _op_size(op):
+    return op.size if isinstance(op, BSE_OPERATOR) else op.size()
+
+
 class DavidsonSolver:
     """DavidsonSolver (davidsonsolver.h:47-334) running on device-resident vectors."""
 
@@ -691,16 +820,77 @@ class DavidsonSolver:
         self.opts.precision_start = _PREC[start] if start is not None else -1
         self.opts.switch_tolerance = float(switch_tolerance)
 
-    def solve(self, op: BSE_OPERATOR, neigen: int, want_vectors=True):
+    def set_matrix_type(self, mt: str):
+        """SYMM or HAM (davidsonsolver.cc:89-103); HAM needs a HamiltonianOperator."""
+        if mt not in ("SYMM", "HAM"):
+            raise RuntimeError(mt + " is not a valid Davidson matrix type")
+        self.matrix_type = mt
+
+    def solve(self, op, neigen: int, want_vectors=True, ctx: Context | None = None, initial_guess=None):
+        """DavidsonSolver::solve(A, neigen): A is a BSE_OPERATOR (the fused device operator), a
+        MatrixFreeOperator (SYMM), or a HamiltonianOperator (HAM, unit 2N-long eigenvectors;
+        `initial_guess` (2N x k) selects the initial-guess overload, davidsonsolver.h:123-152)."""
+        if isinstance(op, HamiltonianOperator):
+            return self._solve_ham(op, neigen, want_vectors, ctx, initial_guess)
+        if getattr(self, "matrix_type", "SYMM") == "HAM":
+            raise RuntimeError("the HAM matrix type needs a HamiltonianOperator")
+        if isinstance(op, MatrixFreeOperator):
+            ctx = ctx or default_context()
+            h = op._handle(ctx)
+            n = op.size()
+            ev = np.zeros(neigen)
+            vec = np.zeros((n, neigen), order="F") if want_vectors else None
+            rc = lib().xtpgpu_davidson_solve_mf(h, neigen, C.byref(self.opts), _dptr(ev),
+                                                _dptr(vec) if want_vectors else None, n, C.byref(self.info))
+            if op._err:
+                err, op._err = op._err[0], []
+                raise err
+            return self._finish(rc, ev, vec)
         ev = np.zeros(neigen)
         vec = np.zeros((op.size, neigen), order="F") if want_vectors else None
         rc = lib().xtpgpu_davidson_solve(op.h, neigen, C.byref(self.opts), _dptr(ev),
                                          _dptr(vec) if want_vectors else None, op.size, C.byref(self.info))
+        return self._finish(rc, ev, vec)
+
+    def _finish(self, rc, ev, vec):
         self._eval, self._evec = ev, vec
         self._converged = rc == 0
         if rc not in (0, -4):
             _check(rc)
         return ev, vec
+
+    def _solve_ham(self, H, neigen, want_vectors, ctx, initial_guess=None):
+        handles, owned = [], []
+        for part in (H.R, H.C):
+            if isinstance(part, BSE_OPERATOR):
+                h = _vp()
+                _check(lib().xtpgpu_mfop_from_bse(part.h, C.byref(h)))
+                owned.append(h)
+                handles.append(h)
+            else:
+                handles.append(part._handle(ctx or (H.R.Mmn.ctx if isinstance(H.R, BSE_OPERATOR)
+                                                    else default_context())))
+        try:
+            n2 = H.rows()
+            ev = np.zeros(neigen)
+            vec = np.zeros((n2, neigen), order="F") if want_vectors else None
+            g = None if initial_guess is None else _f64(initial_guess)
+            if g is not None and g.shape[0] != n2:
+                raise ValueError("initial_guess must have 2N rows")
+            rc = lib().xtpgpu_davidson_solve_ham_mf(handles[0], handles[1], neigen, C.byref(self.opts),
+                                                    _dptr(g) if g is not None else None, n2,
+                                                    g.shape[1] if g is not None else 0, _dptr(ev),
+                                                    _dptr(vec) if want_vectors else None, n2, C.byref(self.info))
+        finally:
+            for h in owned:
+                lib().xtpgpu_mfop_destroy(h)
+        for part in (H.R, H.C):
+            errs = getattr(part, "_err", None)
+            if errs:
+                err = errs[0]
+                part._err = []
+                raise err
+        return self._finish(rc, ev, vec)
 
     def solve_full(self, A: BSE_OPERATOR, B: BSE_OPERATOR, neigen: int, want_vectors=True, initial_guess=None):
         """Full BSE with the HamiltonianOperator [[A,B],[-B,-A]] (BSE::Solve_nonhermitian_Davidson);
