@@ -105,10 +105,39 @@ def test_rpa_random_multi_frequency(ctx, prec, tol):
         assert np.array_equal(g[:, :, i], g[:, :, i].T)
 
 
+@pytest.mark.parametrize("prec,tol", [("bf16x3", 2e-5), ("bf16", 2e-2)])
+@pytest.mark.parametrize("kind,freqs", [("imag", [0.0, 0.3, 1.7]), ("real", [0.0, 0.05, 0.1])])
+def test_rpa_mixed_frequency_groups(ctx, monkeypatch, prec, tol, kind, freqs):
+    """The mixed eps builds with the frequency weights folded into scaled transition copies, one
+    frequency per group (XTPGPU_EPS_SCALE_BYTES=1) and all in one group, against the oracle;
+    the real-axis weights change sign across the transitions."""
+    rng = np.random.default_rng(5)
+    o, v, P = 8, 60, 130
+    M = rng.normal(scale=0.3, size=(o + v, o + v, P))
+    e = np.concatenate([np.sort(rng.uniform(-1, -0.2, o)), np.sort(rng.uniform(0.3, 1.5, v))])
+    rpa = px.RPA(dev_mmn(ctx, M))
+    rpa.configure(o - 1, 0, o + v - 1)
+    rpa.setRPAInputEnergies(e)
+    k = px.EPS_IMAG if kind == "imag" else px.EPS_REAL
+    outs = []
+    for budget in ("1", None):
+        if budget:
+            monkeypatch.setenv("XTPGPU_EPS_SCALE_BYTES", budget)
+        else:
+            monkeypatch.delenv("XTPGPU_EPS_SCALE_BYTES", raising=False)
+        g = rpa.calculate_epsilon(k, freqs, prec)
+        for i, f in enumerate(freqs):
+            ref = X.rpa_epsilon(M, e, o - 1, 0, o + v - 1, f, kind)
+            assert relerr(g[:, :, i], ref) < tol, (prec, kind, f, budget)
+            assert np.array_equal(g[:, :, i], g[:, :, i].T)
+        outs.append(g)
+    assert np.array_equal(outs[0], outs[1])  # grouping does not change a single bit
+
+
 @pytest.mark.parametrize("kern", ["staged32", "staged64"])
 def test_rpa_bf16x3_kernel_variants(ctx, kern, monkeypatch):
-    """eps(i w) through both register-staged BF16 kernels (K step 32 and 64; the frequency
-    weight is applied on the B-operand load) against the oracle."""
+    """eps(i w) through both register-staged BF16 kernels (K step 32 and 64, on the scaled
+    transition copy) against the oracle."""
     rng = np.random.default_rng(3)
     o, v, P = 11, 97, 210
     M = rng.normal(scale=0.3, size=(o + v, o + v, P))

@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 #include <stdint.h>
+#include <cstdlib>
+#include <cstring>
 
 namespace xg {
 

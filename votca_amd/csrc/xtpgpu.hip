@@ -765,6 +765,17 @@ void rpa_pack(xtpgpu_mmn mmn, long n_occ, long n_unocc, bool need64, bool need16
   }
 }
 
+// Bytes of scaled transition copies one mixed-precision eps build may hold at once: 12 GiB, or
+// XTPGPU_EPS_SCALE_BYTES when set (a small value forces one frequency per group).
+size_t eps_scale_budget() {
+  const char* e = std::getenv("XTPGPU_EPS_SCALE_BYTES");
+  if (e && *e) {
+    const long long v = std::atoll(e);
+    if (v > 0) return (size_t)v;
+  }
+  return (size_t)12 << 30;
+}
+
 void rpa_epsilon_impl(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* energies, int64_t homo, int64_t rpamin,
                       int64_t rpamax, int kind, const double* omega_re, const double* omega_im, int n_omega,
                       double eta, int precision, double* out_dev) {
@@ -835,9 +846,53 @@ void rpa_epsilon_impl(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* energies, in
     Opnd<double> A{pk.A64.as<double>(), nullptr, Kp, 0};
     gemm64<true, true>(st, (int)P, (int)P, (int)Kp, n_omega, 1, A, A, d64, Kp, true, epi);
   } else {
+    // the weights go into a scaled copy of the transitions, B_w = M diag(d_w) as BF16 hi (+ lo),
+    // built once per frequency, so the GEMM's inner loop is plain MFMA work; frequencies are
+    // taken in equal groups whose scaled copies fit eps_scale_budget()
     const bool split = precision == XTPGPU_BF16X3;
-    Opnd<unsigned short> A{pk.Ah.as<unsigned short>(), pk.Al.as<unsigned short>(), Kp, 0};
-    gemm16<true, true>(st, (int)P, (int)P, (int)Kp, n_omega, 1, A, A, d32, Kp, split, true, epi);
+    const size_t plane = (size_t)P * Kp;
+    const size_t per_w = plane * 2 * (split ? 2 : 1);
+    const long gmax = std::max<long>(1, std::min<long>(n_omega, (long)(eps_scale_budget() / per_w)));
+    const long ngroups = (n_omega + gmax - 1) / gmax;
+    const long gsz = (n_omega + ngroups - 1) / ngroups;
+    DevBuf sc(per_w * gsz);
+    unsigned short* Bh = sc.as<unsigned short>();
+    unsigned short* Bl = split ? Bh + plane * gsz : nullptr;
+    const unsigned short* Ah = pk.Ah.as<unsigned short>();
+    const unsigned short* Al = pk.Al.as<unsigned short>();
+    const Opnd<unsigned short> A{Ah, Al, Kp, 0};
+    for (long w0 = 0; w0 < n_omega; w0 += gsz) {
+      const long nw_g = std::min<long>(gsz, n_omega - w0);
+      const float* dg = d32 + w0 * Kp;
+      const long Kpl = Kp;
+      const size_t plane8 = plane / 8;  // Kp is a multiple of 8: 8-element vectors stay in one row
+      // one 16-byte vector of transitions per thread, read once and scaled for every frequency
+      // of the group
+      launch_map(st, plane8, [=] __device__(size_t i) {
+        const us8v h = reinterpret_cast<const us8v*>(Ah)[i], l = reinterpret_cast<const us8v*>(Al)[i];
+        float a[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) a[e] = bf16_bits_to_f(h[e]) + bf16_bits_to_f(l[e]);
+        const long k0 = (long)((i * 8) % (size_t)Kpl);
+        for (long f = 0; f < nw_g; ++f) {
+          const float4 d0 = *reinterpret_cast<const float4*>(dg + f * Kpl + k0);
+          const float4 d1 = *reinterpret_cast<const float4*>(dg + f * Kpl + k0 + 4);
+          const float dv[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+          us8v oh, ol;
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const float x = a[e] * dv[e];
+            oh[e] = bf16_bits_rn(x);
+            ol[e] = bf16_bits_rn(x - bf16_bits_to_f(oh[e]));
+          }
+          reinterpret_cast<us8v*>(Bh)[f * plane8 + i] = oh;
+          if (Bl) reinterpret_cast<us8v*>(Bl)[f * plane8 + i] = ol;
+        }
+      });
+      EpiEpsSym eg{out_dev + w0 * P * P, P};
+      gemm16<true, true>(st, (int)P, (int)P, (int)Kp, (int)nw_g, 1, A,
+                         Opnd<unsigned short>{Bh, Bl, Kp, (long)plane}, nullptr, 0, split, true, eg);
+    }
   }
   XG_HIP(hipStreamSynchronize(st));
 }
