@@ -419,6 +419,162 @@ def run_large_eps(args):
                       "algorithmic_flops": F, "sampled_frequencies": sample, **res}))
 
 
+def run_large(args):
+    """The large single segment (default n_occ=250, n_virt=2250, n_aux=6000) on one GPU with no
+    resident Mmn window (the full window, 2500^2 * 6000 doubles = 300 GB, exceeds the device):
+      1. eps(i w) at the 20 Gauss-Legendre points of the CDA default (gwbse.xml:18-19) from the
+         occupied x all slices (30 GB), FP64 and mixed;
+      2. eps_r(0) and its inverse W (host eigendecomposition, P x P);
+      3. the singlet TDA operator packed from the synthetic source in aux chunks (BF16 hi/lo M_cc'
+         resident; the FP64 M_cc' block, 243 GB, streamed from the source in every FP64 product;
+         screening by the full W on the valence block), then the mixed Davidson for `--large-roots`
+         roots with the BF16 ladder and FP64 refinement;
+      4. per-root FP64 residual norms ||H x - w x|| of the returned vectors, with the FP64 operator:
+         |w - w_exact| <= ||r|| bounds each root's error against the FP64 spectrum;
+      5. Sigma_c^CDA (20 Legendre points) at the intercepts of the frontier levels, with the count and
+         cost of the residue eps builds the 100-level QP window needs."""
+    from votca_amd import pyxtp as px
+    from votca_amd import quadrature
+    o, v, P = args.large_occ, args.large_virt, args.large_aux
+    n = o + v
+    N = o * v
+    ctx = px.Context(0)
+    out = {"workload": f"large single segment n_occ={o} n_virt={v} n_aux={P}, streamed M_cc'",
+           "sigma0": args.sigma0, "roots": args.large_roots, "tol": args.large_tol}
+    e = synthetic_energies(args.seed, o, v)
+    # ---- 1. eps(i w): occupied x all slices of the same synthetic tensor
+    t0 = time.perf_counter()
+    occ = px.TCMatrix_gwbse(ctx).Initialize(P, 0, o - 1, 0, n - 1)
+    occ.fill_synthetic(args.seed, args.sigma0, 3.0)
+    log(f"[large] o={o} v={v} P={P}: occupied x all slices {o*n*P*8/1e9:.1f} GB ({time.perf_counter()-t0:.1f} s)")
+    rpa = px.RPA(occ)
+    rpa.configure(o - 1, 0, n - 1)
+    rpa.setRPAInputEnergies(e)
+    nf = 20
+    freqs = np.ascontiguousarray(quadrature.create("legendre", nf).points())
+    F = float(nf) * o * v * P * (P + 1)
+    ctx.set_profiling(True)
+    eps_dev = px.DeviceBuffer(ctx, P * P * nf * 8)
+    eps = {}
+    for prec in ("fp64", args.mixed):
+        ctx.reset_timers()
+        t0 = time.perf_counter()
+        px._check(px.lib().xtpgpu_rpa_epsilon_device(ctx.h, occ.h, px._dptr(e), o - 1, 0, n - 1, px.EPS_IMAG,
+                                                     px._dptr(freqs), None, nf, 1e-4, px._PREC[prec], eps_dev.ptr))
+        ctx.synchronize()
+        tm = ctx.timer("rpa_eps_gemm")
+        eps[prec] = {"wall_s": time.perf_counter() - t0, "kernel_s": tm["ms"] / 1e3,
+                     "tflops": F / (tm["ms"] / 1e3) / 1e12}
+        slab = eps_dev.download((P, P), offset=(nf // 2) * P * P * 8)
+        if prec == "fp64":
+            ref = slab
+        else:
+            eps[prec]["rel_frobenius_err_mid_frequency"] = float(np.linalg.norm(slab - ref) / np.linalg.norm(ref))
+        log(f"[large] eps {prec}: {eps[prec]['kernel_s']:.2f} s ({eps[prec]['tflops']:.1f} TFLOP/s)")
+    eps["speedup"] = eps["fp64"]["kernel_s"] / eps[args.mixed]["kernel_s"]
+    out["eps_20_frequencies"] = eps
+    del eps_dev, ref, slab
+    # ---- 2. W = eps_r(0)^-1
+    t0 = time.perf_counter()
+    lam, U = np.linalg.eigh(rpa.calculate_epsilon_r(0.0))
+    W = (U / lam) @ U.T
+    W = 0.5 * (W + W.T)
+    out["screening_s"] = time.perf_counter() - t0
+    log(f"[large] eps_r(0) and its inverse: {out['screening_s']:.1f} s")
+    del U, lam, rpa, occ
+    # ---- 5 (first, while no operator holds the device): Sigma_c^CDA on the frontier levels
+    if args.large_cda_levels > 0:
+        out["cda"] = run_large_cda(px, ctx, args, e, o, v, P)
+    # ---- 3. streamed operator and the mixed Davidson
+    src = px.SyntheticSource(args.seed, args.sigma0, 3.0, P)
+    t0 = time.perf_counter()
+    op = px.SingletOperator_TDA(None, None, np.diag(e), precisions=("fp64", "bf16", args.mixed)).configure_streamed(
+        px.BSEOperator_Options(o - 1, 0, 0, 0, n - 1), src, ctx, stream_fp64=True, chunk_bytes=4 << 30,
+        epsinv_matrix=W)
+    out["operator"] = {"pack_s": time.perf_counter() - t0, "device_bytes": op.memory_bytes(),
+                       "fp64_mcc_bytes_not_held": P * v * v * 8}
+    log(f"[large] operator packed in {out['operator']['pack_s']:.1f} s, {op.memory_bytes()/1e9:.1f} GB on device")
+    ds = px.DavidsonSolver()
+    ds.set_tolerance(args.large_tol)
+    ds.set_iter_max(args.large_iter)
+    ds.set_max_search_space(10 * args.large_roots)
+    ds.set_precision(args.mixed, refine_tolerance=args.large_tol, refine_max=args.refine_max, start="bf16",
+                     switch_tolerance=args.switch_tol)
+    os.environ["XTPGPU_DAVIDSON_LOG"] = "1"  # one line per iteration on stderr
+    ctx.reset_timers()
+    t0 = time.perf_counter()
+    w, Xv = ds.solve(op, args.large_roots)
+    wall = time.perf_counter() - t0
+    os.environ["XTPGPU_DAVIDSON_LOG"] = "0"
+    info = ds.info
+    out["bse"] = {"wall_s": wall, "iterations": info.iterations, "matvec_cols": info.matvec_columns,
+                  "matvec_cols_fp64": info.matvec_columns_fp64, "refine_rounds": info.refine_rounds,
+                  "max_residual": info.max_residual, "converged": bool(ds.converged()),
+                  "omega_eV": [float(x) * HRT2EV for x in w],
+                  "timers": {k: ctx.timer(k) for k in ("hd_stage1", "hd_stage1_bf16", "hd_stage1_fp64",
+                                                      "hd_stream_fp64")}}
+    log(f"[large] BSE {args.mixed}: {wall:.1f} s, {info.iterations} it, {info.matvec_columns} cols + "
+        f"{info.matvec_columns_fp64} fp64 cols, res {info.max_residual:.1e}, converged {ds.converged()}")
+    # ---- 4. per-root FP64 residuals
+    t0 = time.perf_counter()
+    k = Xv.shape[1]
+    Xd = px.DeviceBuffer(ctx, N * k * 8)
+    Yd = px.DeviceBuffer(ctx, N * k * 8)
+    Xd.upload(np.ascontiguousarray(Xv.T))  # column-major N x k
+    op.matmul_device(Xd, k, Yd, "fp64")
+    Y = Yd.download((N, k))
+    r = np.linalg.norm(Y - Xv * w[None, :], axis=0)
+    out["bse"]["fp64_residual_norm_per_root"] = [float(x) for x in r]
+    out["bse"]["max_abs_omega_err_bound_meV"] = float(r.max() * HRT2EV * 1e3)
+    log(f"[large] FP64 residuals: max {r.max():.2e} Ha ({r.max()*HRT2EV*1e3:.2e} meV bound), "
+        f"{time.perf_counter()-t0:.1f} s")
+    print(json.dumps(out))
+
+
+def run_large_cda(px, ctx, args, e, o, v, P):
+    """Sigma_c^CDA (20 Legendre points, sigma_cda.cc) at the intercepts of `--large-cda-levels` levels
+    on each side of the gap, from the tensor rows the CDA reads (levels 0 .. LUMO + levels - 1 over
+    all n, FP64) and the mixed residue screening; the number of residue eps builds that level set and
+    the 100-level window take, and the time per build."""
+    from votca_amd import quadrature
+    n = o + v
+    L = args.large_cda_levels
+    qpmin, qpmax = o - L, o + L - 1
+    t = px.TCMatrix_gwbse(ctx).Initialize(P, 0, qpmax, 0, n - 1)
+    t.fill_synthetic(args.seed, args.sigma0, 3.0)
+    rpa = px.RPA(t)
+    rpa.configure(o - 1, 0, n - 1)
+    rpa.setRPAInputEnergies(e)
+    cda = px.Sigma_CDA(t, rpa).configure(o - 1, qpmin, qpmax, 0, n - 1, eta=1e-3, alpha=1e-3,
+                                         quadrature_scheme="legendre", order=20, screening_precision=args.mixed)
+    t0 = time.perf_counter()
+    cda.PrepareScreening()
+    prep = time.perf_counter() - t0
+    levels = list(range(2 * L))
+    om = [float(e[qpmin + l]) for l in levels]
+    t0 = time.perf_counter()
+    sig = cda.diag_batch(levels, om)
+    ev = time.perf_counter() - t0
+
+    def residues(lo, hi):  # states with a residue at w = e_l (CalcResiduePrefactor, sigma_cda.cc:63-77)
+        fermi = 0.5 * (e[o - 1] + e[o])
+        cnt = 0
+        for lev in range(lo, hi + 1):
+            w = e[lev]
+            cnt += int(np.sum(((fermi < e) & (e < w)) | ((fermi > e) & (e > w)) | (np.abs(e - w) < 1e-10)))
+        return cnt
+
+    nres = residues(qpmin, qpmax)
+    n100 = residues(o - 50, o + 49)
+    out = {"levels": [qpmin, qpmax], "prepare_s": prep, "eval_s": ev, "residue_builds": nres,
+           "s_per_residue_build": ev / max(nres, 1), "residue_builds_100_level_window": n100,
+           "sigma_c_Ha": [float(x) for x in sig], "screening": args.mixed}
+    log(f"[large] Sigma_c^CDA at {2*L} intercepts: prepare {prep:.1f} s, evaluation {ev:.1f} s over {nres} residue "
+        f"builds; the 100-level window needs {n100} builds per sweep")
+    cda.close()
+    return out
+
+
 def run_adjoint(args):
     """Full (non-TDA) singlet BSE for `--roots` roots, then the Hellmann-Feynman derivatives of the
     lowest `--adj-roots` with respect to the QP energies (all levels) and the transition block
@@ -569,7 +725,7 @@ def main():
     ap.add_argument("--refine-max", type=int, default=8, help="FP64 refinement rounds of the mixed solve")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-gw", action="store_true", help="skip the mixed-precision G0W0 section")
-    ap.add_argument("--workload", default="single", choices=["single", "segments", "matvec", "adjoint", "large-eps"])
+    ap.add_argument("--workload", default="single", choices=["single", "segments", "matvec", "adjoint", "large-eps", "large"])
     ap.add_argument("--nfreq", type=int, default=20, help="large-eps workload: Gauss-Legendre points")
     ap.add_argument("--adj-roots", type=int, default=5)
     ap.add_argument("--fd-levels", type=int, default=3)
@@ -587,6 +743,14 @@ def main():
     ap.add_argument("--segments-precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--skip-fp64-solve", action="store_true")
     ap.add_argument("--fp64-reps", type=int, default=3, help="FP64 GPU solves (median reported)")
+    ap.add_argument("--large-occ", type=int, default=250)
+    ap.add_argument("--large-virt", type=int, default=2250)
+    ap.add_argument("--large-aux", type=int, default=6000)
+    ap.add_argument("--large-roots", type=int, default=50)
+    ap.add_argument("--large-tol", type=float, default=1e-6)
+    ap.add_argument("--large-iter", type=int, default=300)
+    ap.add_argument("--large-cda-levels", type=int, default=2,
+                    help="large workload: Sigma_c^CDA at this many levels on each side of the gap")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -597,6 +761,9 @@ def main():
         return
     if args.workload == "matvec":
         run_matvec(args)
+        return
+    if args.workload == "large":
+        run_large(args)
         return
     if args.workload == "adjoint":
         run_adjoint(args)

@@ -140,6 +140,46 @@ int xtpgpu_bse_create(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* eps0inv, con
                       int64_t cmax, int cqp, int cx, int cd, int cd2, unsigned precision_mask,
                       xtpgpu_bseop* out);
 int xtpgpu_bse_destroy(xtpgpu_bseop op);
+/* A source of three-centre slices by auxiliary chunk, for operators too large for a resident Mmn
+ * window: fills M^P_{mn} for P in [p0, p0 + np), m, n in [lmin, lmax] (tensor indices, level -
+ * rpamin) into dst (device memory, aux-major: dst[(P - p0) * W * W + (m - lmin) * W + (n - lmin)],
+ * W = lmax - lmin + 1), issued on `stream` (the context's hipStream_t); returns 0 on success.
+ * The reference fills the same slices from the AO three-centre integrals (TCMatrix_gwbse::Fill,
+ * threecenter.cc:29-48); a source may do that per chunk, or regenerate synthetic data. */
+typedef int (*xtpgpu_mmn_chunk_fn)(void* user, int64_t p0, int64_t np, int64_t lmin, int64_t lmax,
+                                   double* dst, void* stream);
+/* The library's synthetic generator (xtpgpu_mmn_fill_synthetic's law: element (m, n, P) of a
+ * tensor with `aux` functions is sigma_P * N(seed, P, min(m, n), max(m, n)), sigma_P = sigma0 *
+ * exp(-decay P / aux)) as a chunk source; user points to an xtpgpu_synthetic_source. */
+typedef struct {
+  uint64_t seed;
+  double sigma0, decay;
+  int64_t aux;
+} xtpgpu_synthetic_source;
+int xtpgpu_synthetic_chunk(void* user, int64_t p0, int64_t np, int64_t lmin, int64_t lmax, double* dst,
+                           void* stream);
+/* xtpgpu_bse_create with the operator packed from a chunk source instead of a resident tensor: the
+ * source is asked for the BSE window [vmin, cmax] (as tensor indices vmin - rpamin ...) in aux
+ * chunks of at most chunk_bytes (0: 4 GiB), so no full Mmn window is ever held. With
+ * XTPGPU_BSE_STREAM_FP64 in flags the FP64 M^P_{cc'} block (the largest, aux * ctotal^2 doubles) is
+ * not kept: FP64 products re-read it from the source chunk by chunk (the source must outlive the
+ * operator); the mixed forms use the resident BF16 planes. No counterpart in the reference, whose
+ * TCMatrix_gwbse always holds the window in host memory.
+ * Screening: eps0inv (aux values) as for xtpgpu_bse_create, for slices already rotated to the
+ * eigenbasis of eps(0) (BSE::SetupDirectInteractionOperator, bse.cc:188-204); or, with
+ * epsinv_matrix (aux x aux, column-major; eps0inv ignored), the unrotated slices and the full
+ * inverse dielectric matrix, W_PQ = (eps^-1(0))_PQ, applied to the valence block
+ * (sum_PQ M^P_vv' W_PQ M^Q_cc' = the reference's sum over the rotated basis), so the source needs no
+ * rotation across aux chunks. epsinv_matrix supports the direct term (cd) only, not cd2. */
+enum { XTPGPU_BSE_STREAM_FP64 = 1 };
+int xtpgpu_bse_create_streamed(xtpgpu_ctx ctx, xtpgpu_mmn_chunk_fn source, void* user, int64_t aux,
+                               const double* eps0inv, const double* epsinv_matrix, const double* Hqp,
+                               int64_t ldhqp, int64_t homo,
+                               int64_t rpamin, int64_t qpmin, int64_t vmin, int64_t cmax, int cqp, int cx,
+                               int cd, int cd2, unsigned precision_mask, int flags, int64_t chunk_bytes,
+                               xtpgpu_bseop* out);
+/* Bytes of device memory the operator holds (packed layouts and grow-only scratch). */
+int xtpgpu_bse_memory(xtpgpu_bseop op, int64_t* bytes);
 /* BSE::AdjustHqpSize (bse.cc:147-186), host only: the GW-window QP Hamiltonian Hqp (gw x gw,
  * gw = qpmax - qpmin + 1, leading dim ldh) cut or extended to the BSE window [vmin, cmax]
  * (hs = (homo - vmin + 1) + (cmax - homo) levels): overlapping levels take their Hqp block, a BSE

@@ -950,6 +950,14 @@ struct xtpgpu_bseop_s {
   // scratch
   DevBuf xp, xh, xl, s1, s1h, s1l, part, wbuf;
   size_t s1_bytes = 0;
+  // streamed FP64 M_cc' (xtpgpu_bse_create_streamed with XTPGPU_BSE_STREAM_FP64): the source,
+  // the tensor index of the first conduction level, and the chunk scratch (source window, packed)
+  xtpgpu_mmn_chunk_fn src_fn = nullptr;
+  void* src_user = nullptr;
+  bool stream64 = false;
+  bool screen_full = false;  // Bvv carries the full eps^-1(0) matrix (unrotated aux basis)
+  long cmin_r = 0;
+  DevBuf srcbuf, mcc64;
 };
 
 namespace {
@@ -1140,10 +1148,27 @@ void bse_matmul_impl(xtpgpu_bseop op, int precision, const double* X, long ldx, 
       if (!m16) {
         double* S = reinterpret_cast<double*>(s1a);
         e1.S64 = S;
+        const double* mcc = op->stream64 ? nullptr : op->Mcc.as<double>() + p0 * ct * ctp;
+        if (op->stream64) {
+          // this chunk's M^P_{c c'} from the source, packed to [p][c1][ctp]
+          ProfScope prof(pctx, "hd_stream_fp64", 0.0, 8.0 * 2.0 * np * ct * (double)(ct + ctp));
+          op->srcbuf.ensure((size_t)np * ct * ct * 8);
+          op->mcc64.ensure((size_t)np * ct * ctp * 8);
+          double* W = op->srcbuf.as<double>();
+          double* A = op->mcc64.as<double>();
+          const int rc = op->src_fn(op->src_user, p0, np, op->cmin_r, op->cmin_r + ct - 1, W, (void*)st);
+          XG_REQUIRE(rc == 0, rc < 0 ? rc : XTPGPU_ERR_ARG, "bse matmul: the Mmn chunk source failed");
+          XG_HIP(hipGetLastError());
+          launch_map(st, (size_t)np * ct * ctp, [=] __device__(size_t i) {
+            const long q = (long)(i / (size_t)(ct * ctp)), r = (long)(i % (size_t)(ct * ctp));
+            const long c1 = r / ctp, c2 = r % ctp;
+            A[i] = c2 < ct ? W[(q * ct + c1) * ct + c2] : 0.0;
+          });
+          mcc = A;
+        }
         {
           ProfScope prof(pctx, "hd_stage1_fp64", f1, b1);
-          gemm64<true, true>(st, (int)M1, (int)N1, (int)round8(ct), 1, 1,
-                             Opnd<double>{op->Mcc.as<double>() + p0 * ct * ctp, nullptr, ctp, 0},
+          gemm64<true, true>(st, (int)M1, (int)N1, (int)round8(ct), 1, 1, Opnd<double>{mcc, nullptr, ctp, 0},
                              Opnd<double>{Xp, nullptr, ctp, 0}, nullptr, 0, false, e1);
         }
         ProfScope prof(pctx, "hd_stage2_fp64", f2, b2);
@@ -1292,6 +1317,45 @@ extern "C" int xtpgpu_bse_adjust_hqp(const double* Hqp, int64_t ldh, const doubl
   });
 }
 
+namespace {
+
+// The BSE-sized Hqp blocks (bse.cc:147-186) and the screening weights eps0^-1 of an operator whose
+// vt, ct, P and cqp are set.
+void setup_hqp_eps(xtpgpu_bseop op, const double* Hqp, int64_t ldhqp, const double* eps0inv) {
+  hipStream_t st = op->ctx->stream;
+  const long vt = op->vt, ct = op->ct, P = op->P, hs = vt + ct;
+  if (op->cqp) {
+    XG_REQUIRE(ldhqp >= hs, XTPGPU_ERR_SHAPE, "xtpgpu_bse_create: ldhqp < vtotal+ctotal");
+    std::vector<double> hc(ct * ct), hv(vt * vt), hcd(ct), hvd(vt);
+    bool diag = true;
+    for (long j = 0; j < hs; ++j)
+      for (long i = 0; i < hs; ++i)
+        if (i != j && Hqp[j * ldhqp + i] != 0.0) diag = false;
+    for (long j = 0; j < ct; ++j)
+      for (long i = 0; i < ct; ++i) hc[j * ct + i] = Hqp[(vt + j) * ldhqp + vt + i];
+    for (long j = 0; j < vt; ++j)
+      for (long i = 0; i < vt; ++i) hv[j * vt + i] = Hqp[j * ldhqp + i];
+    for (long i = 0; i < ct; ++i) hcd[i] = Hqp[(vt + i) * ldhqp + vt + i];
+    for (long i = 0; i < vt; ++i) hvd[i] = Hqp[i * ldhqp + i];
+    op->hqp_diag = diag;
+    op->hc.alloc(hc.size() * 8); op->hv.alloc(hv.size() * 8);
+    op->hcd.alloc(ct * 8); op->hvd.alloc(vt * 8);
+    XG_HIP(hipMemcpyAsync(op->hc.p, hc.data(), hc.size() * 8, hipMemcpyHostToDevice, st));
+    XG_HIP(hipMemcpyAsync(op->hv.p, hv.data(), hv.size() * 8, hipMemcpyHostToDevice, st));
+    XG_HIP(hipMemcpyAsync(op->hcd.p, hcd.data(), ct * 8, hipMemcpyHostToDevice, st));
+    XG_HIP(hipMemcpyAsync(op->hvd.p, hvd.data(), vt * 8, hipMemcpyHostToDevice, st));
+    XG_HIP(hipStreamSynchronize(st));
+  }
+  op->eps.alloc(P * 8);
+  if (eps0inv) {
+    XG_HIP(hipMemcpyAsync(op->eps.p, eps0inv, P * 8, hipMemcpyHostToDevice, st));
+  } else {
+    fill_zero(st, op->eps.as<double>(), P);
+  }
+}
+
+}  // namespace
+
 extern "C" int xtpgpu_bse_create(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* eps0inv, const double* Hqp,
                                  int64_t ldhqp, int64_t homo, int64_t rpamin, int64_t qpmin, int64_t vmin,
                                  int64_t cmax, int cqp, int cx, int cd, int cd2, unsigned precision_mask,
@@ -1322,36 +1386,7 @@ extern "C" int xtpgpu_bse_create(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* e
     hipStream_t st = ctx->stream;
     const double* D = mmn->d();
 
-    // Hqp blocks (BSE-sized, bse.cc:147-186)
-    const long hs = vt + ct;
-    if (cqp) {
-      XG_REQUIRE(ldhqp >= hs, XTPGPU_ERR_SHAPE, "xtpgpu_bse_create: ldhqp < vtotal+ctotal");
-      std::vector<double> hc(ct * ct), hv(vt * vt), hcd(ct), hvd(vt);
-      bool diag = true;
-      for (long j = 0; j < hs; ++j)
-        for (long i = 0; i < hs; ++i)
-          if (i != j && Hqp[j * ldhqp + i] != 0.0) diag = false;
-      for (long j = 0; j < ct; ++j)
-        for (long i = 0; i < ct; ++i) hc[j * ct + i] = Hqp[(vt + j) * ldhqp + vt + i];
-      for (long j = 0; j < vt; ++j)
-        for (long i = 0; i < vt; ++i) hv[j * vt + i] = Hqp[j * ldhqp + i];
-      for (long i = 0; i < ct; ++i) hcd[i] = Hqp[(vt + i) * ldhqp + vt + i];
-      for (long i = 0; i < vt; ++i) hvd[i] = Hqp[i * ldhqp + i];
-      op->hqp_diag = diag;
-      op->hc.alloc(hc.size() * 8); op->hv.alloc(hv.size() * 8);
-      op->hcd.alloc(ct * 8); op->hvd.alloc(vt * 8);
-      XG_HIP(hipMemcpyAsync(op->hc.p, hc.data(), hc.size() * 8, hipMemcpyHostToDevice, st));
-      XG_HIP(hipMemcpyAsync(op->hv.p, hv.data(), hv.size() * 8, hipMemcpyHostToDevice, st));
-      XG_HIP(hipMemcpyAsync(op->hcd.p, hcd.data(), ct * 8, hipMemcpyHostToDevice, st));
-      XG_HIP(hipMemcpyAsync(op->hvd.p, hvd.data(), vt * 8, hipMemcpyHostToDevice, st));
-      XG_HIP(hipStreamSynchronize(st));
-    }
-    op->eps.alloc(P * 8);
-    if (eps0inv) {
-      XG_HIP(hipMemcpyAsync(op->eps.p, eps0inv, P * 8, hipMemcpyHostToDevice, st));
-    } else {
-      fill_zero(st, op->eps.as<double>(), P);
-    }
+    setup_hqp_eps(op.get(), Hqp, ldhqp, eps0inv);
     const double* E = op->eps.as<double>();
     const bool need16 = ((op->mask >> XTPGPU_BF16) & 1u) || ((op->mask >> XTPGPU_BF16X3) & 1u);
 
@@ -1447,6 +1482,245 @@ extern "C" int xtpgpu_bse_create(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* e
   });
 }
 
+extern "C" int xtpgpu_synthetic_chunk(void* user, int64_t p0, int64_t np, int64_t lmin, int64_t lmax, double* dst,
+                                      void* stream) {
+  return guarded([&] {
+    XG_REQUIRE(user && dst && np >= 0 && lmax >= lmin && p0 >= 0, XTPGPU_ERR_ARG, "xtpgpu_synthetic_chunk: bad arguments");
+    const xtpgpu_synthetic_source src = *static_cast<const xtpgpu_synthetic_source*>(user);
+    XG_REQUIRE(p0 + np <= src.aux, XTPGPU_ERR_SHAPE, "xtpgpu_synthetic_chunk: aux range beyond the source");
+    if (np == 0) return;
+    // the same sigma table as xtpgpu_mmn_fill_synthetic (host exp, so the values match bit for bit)
+    std::vector<double> sig(np);
+    for (long q = 0; q < np; ++q)
+      sig[q] = src.sigma0 * std::exp(-src.decay * (double)(p0 + q) / (double)src.aux);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    DevBuf ds(np * sizeof(double));
+    XG_HIP(hipMemcpyAsync(ds.p, sig.data(), np * sizeof(double), hipMemcpyHostToDevice, st));
+    const double* sg = ds.as<double>();
+    const long W = lmax - lmin + 1, l0 = lmin, pp0 = p0;
+    const uint64_t seed = src.seed;
+    launch_map(st, (size_t)np * W * W, [=] __device__(size_t i) {
+      const long q = (long)(i / (size_t)(W * W)), r = (long)(i % (size_t)(W * W));
+      const long m = r / W + l0, n = r % W + l0;
+      const long a = m < n ? m : n, b = m < n ? n : m;
+      dst[i] = synth_normal(seed, (uint64_t)(pp0 + q), (uint64_t)a, (uint64_t)b) * sg[q];
+    });
+    XG_HIP(hipStreamSynchronize(st));  // ds is freed on return
+  });
+}
+
+extern "C" int xtpgpu_bse_create_streamed(xtpgpu_ctx ctx, xtpgpu_mmn_chunk_fn source, void* user, int64_t aux,
+                                          const double* eps0inv, const double* epsinv_matrix, const double* Hqp,
+                                          int64_t ldhqp, int64_t homo,
+                                          int64_t rpamin, int64_t qpmin, int64_t vmin, int64_t cmax, int cqp,
+                                          int cx, int cd, int cd2, unsigned precision_mask, int flags,
+                                          int64_t chunk_bytes, xtpgpu_bseop* out) {
+  return guarded([&] {
+    (void)qpmin;
+    XG_REQUIRE(ctx && source && out, XTPGPU_ERR_ARG, "xtpgpu_bse_create_streamed: null argument");
+    XG_REQUIRE(aux > 0, XTPGPU_ERR_SHAPE, "xtpgpu_bse_create_streamed: aux must be positive");
+    XG_REQUIRE(!(cd != 0 && cd2 != 0), XTPGPU_ERR_ARG, "Hamiltonian cannot contain Hd and Hd2 at the same time");
+    XG_REQUIRE(cqp == 0 || Hqp, XTPGPU_ERR_ARG, "xtpgpu_bse_create: Hqp required when cqp != 0");
+    XG_REQUIRE((cd == 0 && cd2 == 0) || eps0inv || epsinv_matrix, XTPGPU_ERR_ARG,
+               "xtpgpu_bse_create: eps0inv required for Hd/Hd2");
+    XG_REQUIRE(!(epsinv_matrix && cd2), XTPGPU_ERR_ARG,
+               "xtpgpu_bse_create_streamed: the full inverse dielectric matrix supports the direct term only");
+    XG_REQUIRE(vmin >= rpamin && homo >= vmin && cmax > homo, XTPGPU_ERR_SHAPE, "xtpgpu_bse_create_streamed: empty BSE window");
+    auto op = std::make_unique<xtpgpu_bseop_s>();
+    op->ctx = ctx;
+    op->cqp = cqp; op->cx = cx; op->cd = cd; op->cd2 = cd2;
+    op->mask = (precision_mask | 1u) & ((1u << XTPGPU_FP64) | (1u << XTPGPU_BF16) | (1u << XTPGPU_BF16X3));
+    const long vt = homo - vmin + 1, ct = cmax - homo;
+    op->vt = vt; op->ct = ct; op->N = vt * ct; op->P = aux;
+    op->ctp = ct >= 64 ? (ct + 63) / 64 * 64 : round8(ct); op->vtp = round8(vt); op->Pp = round8(op->P);
+    const long P = op->P, N = op->N, ctp = op->ctp, vtp = op->vtp, Pp = op->Pp, Np = vt * ctp;
+    XG_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    // with the full matrix the valence block is packed unscaled (weights 1) and multiplied by W later
+    std::vector<double> ones;
+    if (epsinv_matrix) ones.assign(P, 1.0);
+    setup_hqp_eps(op.get(), Hqp, ldhqp, epsinv_matrix ? ones.data() : eps0inv);
+    const double* E = op->eps.as<double>();
+    const bool need16 = ((op->mask >> XTPGPU_BF16) & 1u) || ((op->mask >> XTPGPU_BF16X3) & 1u);
+    const bool keep_mcc64 = !(flags & XTPGPU_BSE_STREAM_FP64);
+    // the source window is the BSE window: tensor indices [w0, w0 + W); v = w0 + v1, c = w0 + vt + c1
+    const long w0 = vmin - rpamin, W = vt + ct;
+    op->cmin_r = w0 + vt;
+    if (!keep_mcc64 && cd) {
+      op->stream64 = true;
+      op->src_fn = source;
+      op->src_user = user;
+    }
+    // layouts (pads zero)
+    if (cd) {
+      if (keep_mcc64) op->Mcc.alloc((size_t)P * ct * ctp * 8);
+      op->Bvv.alloc((size_t)vt * P * vtp * 8);
+      zero_bytes(st, op->Bvv.p, op->Bvv.bytes);
+      if (need16) {
+        op->Mcc_h.alloc((size_t)P * ct * ctp * 2); op->Mcc_l.alloc((size_t)P * ct * ctp * 2);
+        op->Bvv_h.alloc((size_t)vt * P * vtp * 2); op->Bvv_l.alloc((size_t)vt * P * vtp * 2);
+      }
+    }
+    if (cx || cd2) {
+      op->Mvc.alloc((size_t)P * Np * 8);
+      if (need16 && cd2) { op->Mvc_h.alloc((size_t)P * Np * 2); op->Mvc_l.alloc((size_t)P * Np * 2); }
+    }
+    if (cx) {
+      op->MvcT.alloc((size_t)N * Pp * 8);
+      zero_bytes(st, op->MvcT.p, op->MvcT.bytes);
+    }
+    if (cd2) {
+      op->McvT.alloc((size_t)ct * P * vtp * 8);
+      zero_bytes(st, op->McvT.p, op->McvT.bytes);
+      if (need16) { op->McvT_h.alloc((size_t)ct * P * vtp * 2); op->McvT_l.alloc((size_t)ct * P * vtp * 2); }
+    }
+    op->diag.alloc((size_t)N * 8);
+    double* dg = op->diag.as<double>();
+    fill_zero(st, dg, N);
+    // M^P_{cc} (diagonal of the conduction block) for the direct term's share of the diagonal
+    DevBuf mccd(cd ? (size_t)P * ct * 8 : 0);
+    double* Mccd = cd ? mccd.as<double>() : nullptr;
+    // aux chunks of the source window
+    const size_t budget = chunk_bytes > 0 ? (size_t)chunk_bytes : ((size_t)4 << 30);
+    const long pc = std::max<long>(1, std::min<long>(P, (long)(budget / ((size_t)W * W * 8))));
+    DevBuf win((size_t)pc * W * W * 8);
+    double* Dc = win.as<double>();
+    double* Mcc = keep_mcc64 && cd ? op->Mcc.as<double>() : nullptr;
+    unsigned short* Mch = need16 && cd ? op->Mcc_h.as<unsigned short>() : nullptr;
+    unsigned short* Mcl = need16 && cd ? op->Mcc_l.as<unsigned short>() : nullptr;
+    double* Bvv = cd ? op->Bvv.as<double>() : nullptr;
+    double* Mvc = (cx || cd2) ? op->Mvc.as<double>() : nullptr;
+    double* MvcT = cx ? op->MvcT.as<double>() : nullptr;
+    double* McvT = cd2 ? op->McvT.as<double>() : nullptr;
+    const double fx = cx, fd = cd, fd2 = cd2;
+    for (long p0 = 0; p0 < P; p0 += pc) {
+      const long np = std::min(pc, P - p0);
+      const int rc = source(user, p0, np, w0, w0 + W - 1, Dc, (void*)st);
+      XG_REQUIRE(rc == 0, rc < 0 ? rc : XTPGPU_ERR_ARG, "xtpgpu_bse_create_streamed: the Mmn chunk source failed");
+      XG_HIP(hipGetLastError());
+      // element (p, a, b) of the chunk, a and b window indices (valence 0..vt-1, conduction vt..W-1)
+      auto el = [=] __device__(long p, long a, long b) { return Dc[((p - p0) * W + a) * W + b]; };
+      if (cd) {
+        launch_map(st, (size_t)np * ct * ctp, [=] __device__(size_t i) {
+          const long p = p0 + (long)(i / (size_t)(ct * ctp)), r = (long)(i % (size_t)(ct * ctp));
+          const long c1 = r / ctp, c2 = r % ctp;
+          const double x = c2 < ct ? el(p, vt + c1, vt + c2) : 0.0;
+          const size_t o = (size_t)p * ct * ctp + r;
+          if (Mcc) Mcc[o] = x;
+          if (Mch) {
+            unsigned short h, l;
+            split_bf16(x, h, l);
+            Mch[o] = h;
+            Mcl[o] = l;
+          }
+        });
+        launch_map(st, (size_t)vt * np * vt, [=] __device__(size_t i) {
+          const long v1 = (long)(i / (size_t)(np * vt)), r = (long)(i % (size_t)(np * vt));
+          const long p = p0 + r / vt, v2 = r % vt;
+          Bvv[(v1 * P + p) * vtp + v2] = E[p] * el(p, v1, v2);
+        });
+        launch_map(st, (size_t)np * ct, [=] __device__(size_t i) {
+          const long p = p0 + (long)(i / (size_t)ct), c = (long)(i % (size_t)ct);
+          Mccd[p * ct + c] = el(p, vt + c, vt + c);
+        });
+      }
+      if (Mvc) {
+        launch_map(st, (size_t)np * Np, [=] __device__(size_t i) {
+          const long p = p0 + (long)(i / (size_t)Np), r = (long)(i % (size_t)Np);
+          const long v = r / ctp, c = r % ctp;
+          Mvc[(size_t)p * Np + r] = c < ct ? el(p, v, vt + c) : 0.0;
+        });
+      }
+      if (MvcT) {
+        launch_map(st, (size_t)N * np, [=] __device__(size_t i) {
+          const long r = (long)(i / (size_t)np), p = p0 + (long)(i % (size_t)np);
+          const long v = r / ct, c = r % ct;
+          MvcT[r * Pp + p] = el(p, v, vt + c);
+        });
+      }
+      if (McvT) {
+        launch_map(st, (size_t)ct * np * vt, [=] __device__(size_t i) {
+          const long c1 = (long)(i / (size_t)(np * vt)), r = (long)(i % (size_t)(np * vt));
+          const long p = p0 + r / vt, v2 = r % vt;
+          McvT[(c1 * P + p) * vtp + v2] = el(p, vt + c1, v2);
+        });
+      }
+      // BSE_OPERATOR::diagonal (bse_operator.cc:134-175), this chunk's share of the aux sums
+      launch_map(st, (size_t)N, [=] __device__(size_t i) {
+        const long v = (long)i / ct, c = (long)i % ct;
+        double e = 0.0;
+        if (fx != 0.0) {
+          double sx = 0.0;
+          for (long p = p0; p < p0 + np; ++p) { const double m = el(p, v, vt + c); sx += m * m; }
+          e += fx * sx;
+        }
+        if (fd2 != 0.0) {
+          double sd = 0.0;
+          for (long p = p0; p < p0 + np; ++p) sd += el(p, vt + c, v) * E[p] * el(p, v, vt + c);
+          e -= fd2 * sd;
+        }
+        dg[i] += e;
+      });
+    }
+    op->screen_full = epsinv_matrix != nullptr;
+    if (cd && epsinv_matrix) {
+      // Bvv_v1 (P x vtp, rows P) <- W Bvv_v1 for every v1: the screened valence block in the
+      // unrotated aux basis
+      DevBuf wm((size_t)P * P * 8), braw((size_t)vt * P * vtp * 8);
+      XG_HIP(hipMemcpyAsync(wm.p, epsinv_matrix, (size_t)P * P * 8, hipMemcpyHostToDevice, st));
+      XG_HIP(hipMemcpyAsync(braw.p, Bvv, (size_t)vt * P * vtp * 8, hipMemcpyDeviceToDevice, st));
+      // per v1: C(j, q) = sum_p Braw(p, j) W(q, p), with Braw read M-contiguous (element (p, j) at
+      // p * vtp + j) and W's column q read as row q (W = eps^-1(0) is symmetric)
+      gemm64<false, true>(st, (int)vtp, (int)P, (int)P, (int)vt, 1,
+                          Opnd<double>{braw.as<double>(), nullptr, vtp, P * vtp},
+                          Opnd<double>{wm.as<double>(), nullptr, P, 0}, nullptr, 0, false,
+                          EpiStore{Bvv, 1, vtp, P * vtp, 1.0, 0.0});
+      XG_HIP(hipStreamSynchronize(st));
+    }
+    if (cd) {
+      // direct term of the diagonal: sum_P M^P_{cc} Bvv[v][P][v]
+      const double fdd = cd;
+      launch_map(st, (size_t)N, [=] __device__(size_t i) {
+        const long v = (long)i / ct, c = (long)i % ct;
+        double sd = 0.0;
+        for (long p = 0; p < P; ++p) sd += Mccd[p * ct + c] * Bvv[(v * P + p) * vtp + v];
+        dg[i] -= fdd * sd;
+      });
+    }
+    if (need16 && cd) {
+      split_to_bf16(st, Bvv, op->Bvv_h.as<unsigned short>(), op->Bvv_l.as<unsigned short>(), (size_t)vt * P * vtp);
+    }
+    if (need16 && cd2) {
+      split_to_bf16(st, Mvc, op->Mvc_h.as<unsigned short>(), op->Mvc_l.as<unsigned short>(), (size_t)P * Np);
+      split_to_bf16(st, McvT, op->McvT_h.as<unsigned short>(), op->McvT_l.as<unsigned short>(), (size_t)ct * P * vtp);
+    }
+    if (cqp) {
+      const double* hcd = op->hcd.as<double>();
+      const double* hvd = op->hvd.as<double>();
+      const double fqp = cqp;
+      launch_map(st, (size_t)N, [=] __device__(size_t i) {
+        const long v = (long)i / ct, c = (long)i % ct;
+        dg[i] += fqp * (hcd[c] - hvd[v]);
+      });
+    }
+    XG_HIP(hipStreamSynchronize(st));
+    *out = op.release();
+  });
+}
+
+extern "C" int xtpgpu_bse_memory(xtpgpu_bseop op, int64_t* bytes) {
+  return guarded([&] {
+    XG_REQUIRE(op && bytes, XTPGPU_ERR_ARG, "null argument");
+    const DevBuf* all[] = {&op->hc, &op->hv, &op->hcd, &op->hvd, &op->eps, &op->diag, &op->Mcc, &op->Bvv,
+                           &op->Mvc, &op->MvcT, &op->McvT, &op->Mcc_h, &op->Mcc_l, &op->Bvv_h, &op->Bvv_l,
+                           &op->Mvc_h, &op->Mvc_l, &op->McvT_h, &op->McvT_l, &op->xp, &op->xh, &op->xl,
+                           &op->s1, &op->s1h, &op->s1l, &op->part, &op->wbuf, &op->srcbuf, &op->mcc64};
+    size_t t = 0;
+    for (const DevBuf* b : all) t += b->bytes;
+    *bytes = (int64_t)t;
+  });
+}
+
 extern "C" int xtpgpu_bse_destroy(xtpgpu_bseop op) {
   return guarded([&] {
     if (!op) return;
@@ -1490,6 +1764,9 @@ extern "C" int xtpgpu_bse_matmul_device_noalloc(xtpgpu_bseop op, int precision, 
                "xtpgpu_bse_matmul_device_noalloc: bad leading dimension");
     XG_REQUIRE(precision == XTPGPU_FP64 || ((op->mask >> precision) & 1u), XTPGPU_ERR_ARG,
                "bse matmul: operator was created without this precision");
+    XG_REQUIRE(!(op->stream64 && precision == XTPGPU_FP64), XTPGPU_ERR_ARG,
+               "xtpgpu_bse_matmul_device_noalloc: an operator that streams its FP64 M_cc' from a source "
+               "needs chunk buffers; use xtpgpu_bse_matmul_device");
     const MatmulPlan pl = plan_matmul(op, precision, k);
     const size_t fixed = ScratchArena::up(pl.xp) + ScratchArena::up(pl.xh) + ScratchArena::up(pl.xl) +
                          ScratchArena::up(pl.part) + ScratchArena::up(pl.wbuf) + ScratchArena::up(pl.per_p) + 1024;

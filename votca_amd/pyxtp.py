@@ -202,6 +202,11 @@ SIGNATURES = {
                                      C.c_double, C.c_int, _dp]),
     "xtpgpu_rpa_epsilon_device": (C.c_int, [_vp, _vp, _dp, _i64, _i64, _i64, C.c_int, _dp, _dp, C.c_int,
                                             C.c_double, C.c_int, _vp]),
+    "xtpgpu_synthetic_chunk": (C.c_int, [C.c_void_p, _i64, _i64, _i64, _i64, _vp, C.c_void_p]),
+    "xtpgpu_bse_create_streamed": (C.c_int, [_vp, C.c_void_p, C.c_void_p, _i64, _dp, _dp, _dp, _i64, _i64, _i64,
+                                             _i64, _i64, _i64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint,
+                                             C.c_int, _i64, C.POINTER(_vp)]),
+    "xtpgpu_bse_memory": (C.c_int, [_vp, C.POINTER(_i64)]),
     "xtpgpu_bse_create": (C.c_int, [_vp, _vp, _dp, _dp, _i64, _i64, _i64, _i64, _i64, _i64,
                                     C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint, C.POINTER(_vp)]),
     "xtpgpu_bse_destroy": (C.c_int, [_vp]),
@@ -556,6 +561,49 @@ class BSEOperator_Options:
     cmax: int
 
 
+class _SyntheticSourceC(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("sigma0", C.c_double), ("decay", C.c_double), ("aux", C.c_int64)]
+
+
+_CHUNK_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p)
+
+
+class SyntheticSource:
+    """The library's synthetic Mmn generator as an aux-chunk source (xtpgpu_synthetic_chunk): the
+    values TCMatrix_gwbse.fill_synthetic(seed, sigma0, decay) would hold, produced chunk by chunk."""
+
+    def __init__(self, seed: int, sigma0: float, decay: float, aux: int):
+        self.params = _SyntheticSourceC(int(seed), float(sigma0), float(decay), int(aux))
+        self.aux = int(aux)
+        self.fn = C.cast(lib().xtpgpu_synthetic_chunk, C.c_void_p)
+        self.user = C.cast(C.pointer(self.params), C.c_void_p)
+        self.err = []
+
+
+class CallbackSource:
+    """An aux-chunk source from a Python function fn(p0, np, lmin, lmax) -> array (np, W, W) of
+    M^P_{mn} (W = lmax - lmin + 1, tensor indices); the chunk is copied to the device."""
+
+    def __init__(self, ctx: "Context", fn, aux: int):
+        self.ctx, self.aux, self.err = ctx, int(aux), []
+
+        def cb(_user, p0, np_, lmin, lmax, dst, _stream):
+            try:
+                w = lmax - lmin + 1
+                a = np.ascontiguousarray(fn(int(p0), int(np_), int(lmin), int(lmax)), dtype=np.float64)
+                if a.shape != (np_, w, w):
+                    raise ValueError(f"chunk source returned {a.shape}, expected {(np_, w, w)}")
+                _check(lib().xtpgpu_memcpy_h2d(self.ctx.h, C.c_void_p(dst), a.ctypes.data_as(_vp), a.nbytes))
+                return 0
+            except Exception as ex:  # reported through the status code
+                self.err.append(ex)
+                return -5
+
+        self._cb = _CHUNK_FN(cb)
+        self.fn = C.cast(self._cb, C.c_void_p)
+        self.user = None
+
+
 class BSE_OPERATOR:
     """BSE_OPERATOR<cqp,cx,cd,cd2> behind the MatrixFreeOperator interface
     (bse_operator.h:40-87, matrixfreeoperator.h:45-70): rows(), diagonal(), matmul(X), A @ X."""
@@ -588,6 +636,41 @@ class BSE_OPERATOR:
         self.vt = opt.homo - opt.vmin + 1
         self.ct = opt.cmax - opt.homo
         return self
+
+    def configure_streamed(self, opt: BSEOperator_Options, source, ctx: "Context", stream_fp64=False,
+                           chunk_bytes=0, epsinv_matrix=None):
+        """configure() from an aux-chunk source (SyntheticSource / CallbackSource) instead of a
+        resident tensor (xtpgpu_bse_create_streamed); stream_fp64 keeps no FP64 M_cc' and re-reads
+        it from the source in every FP64 product (the source must outlive the operator). With
+        epsinv_matrix (aux x aux, symmetric) the slices are taken unrotated and screened by the full
+        inverse dielectric matrix instead of the diagonal epsilon_0_inv."""
+        self.close()
+        h = _vp()
+        wm = None if epsinv_matrix is None else _f64(epsinv_matrix)
+        eps = _dptr(self.eps) if self.eps is not None else None
+        hq = _dptr(self.Hqp) if self.Hqp is not None else None
+        ld = self.Hqp.shape[0] if self.Hqp is not None else 0
+        self.source = source
+        rc = lib().xtpgpu_bse_create_streamed(ctx.h, source.fn, source.user, source.aux, eps,
+                                              _dptr(wm) if wm is not None else None, hq, ld, opt.homo,
+                                              opt.rpamin, opt.qpmin, opt.vmin, opt.cmax, *self.c, self.mask,
+                                              1 if stream_fp64 else 0, int(chunk_bytes), C.byref(h))
+        if source.err:
+            err, source.err[:] = source.err[0], []
+            raise err
+        _check(rc)
+        self.h = h
+        n = _i64(0)
+        _check(lib().xtpgpu_bse_size(h, C.byref(n)))
+        self.size = n.value
+        self.vt = opt.homo - opt.vmin + 1
+        self.ct = opt.cmax - opt.homo
+        return self
+
+    def memory_bytes(self) -> int:
+        n = _i64(0)
+        _check(lib().xtpgpu_bse_memory(self.h, C.byref(n)))
+        return n.value
 
     def rows(self):
         return self.size
