@@ -440,12 +440,12 @@ def run_large(args):
     N = o * v
     ctx = px.Context(0)
     out = {"workload": f"large single segment n_occ={o} n_virt={v} n_aux={P}, streamed M_cc'",
-           "sigma0": args.sigma0, "roots": args.large_roots, "tol": args.large_tol}
+           "sigma0": args.large_sigma0, "roots": args.large_roots, "tol": args.large_tol}
     e = synthetic_energies(args.seed, o, v)
     # ---- 1. eps(i w): occupied x all slices of the same synthetic tensor
     t0 = time.perf_counter()
     occ = px.TCMatrix_gwbse(ctx).Initialize(P, 0, o - 1, 0, n - 1)
-    occ.fill_synthetic(args.seed, args.sigma0, 3.0)
+    occ.fill_synthetic(args.seed, args.large_sigma0, 3.0)
     log(f"[large] o={o} v={v} P={P}: occupied x all slices {o*n*P*8/1e9:.1f} GB ({time.perf_counter()-t0:.1f} s)")
     rpa = px.RPA(occ)
     rpa.configure(o - 1, 0, n - 1)
@@ -486,7 +486,7 @@ def run_large(args):
     if args.large_cda_levels > 0:
         out["cda"] = run_large_cda(px, ctx, args, e, o, v, P)
     # ---- 3. streamed operator and the mixed Davidson
-    src = px.SyntheticSource(args.seed, args.sigma0, 3.0, P)
+    src = px.SyntheticSource(args.seed, args.large_sigma0, 3.0, P)
     t0 = time.perf_counter()
     op = px.SingletOperator_TDA(None, None, np.diag(e), precisions=("fp64", "bf16", args.mixed)).configure_streamed(
         px.BSEOperator_Options(o - 1, 0, 0, 0, n - 1), src, ctx, stream_fp64=True, chunk_bytes=4 << 30,
@@ -497,7 +497,7 @@ def run_large(args):
     ds = px.DavidsonSolver()
     ds.set_tolerance(args.large_tol)
     ds.set_iter_max(args.large_iter)
-    ds.set_max_search_space(10 * args.large_roots)
+    ds.set_max_search_space(args.large_space_factor * args.large_roots)
     ds.set_precision(args.mixed, refine_tolerance=args.large_tol, refine_max=args.refine_max, start="bf16",
                      switch_tolerance=args.switch_tol)
     os.environ["XTPGPU_DAVIDSON_LOG"] = "1"  # one line per iteration on stderr
@@ -541,7 +541,7 @@ def run_large_cda(px, ctx, args, e, o, v, P):
     L = args.large_cda_levels
     qpmin, qpmax = o - L, o + L - 1
     t = px.TCMatrix_gwbse(ctx).Initialize(P, 0, qpmax, 0, n - 1)
-    t.fill_synthetic(args.seed, args.sigma0, 3.0)
+    t.fill_synthetic(args.seed, args.large_sigma0, 3.0)
     rpa = px.RPA(t)
     rpa.configure(o - 1, 0, n - 1)
     rpa.setRPAInputEnergies(e)
@@ -578,10 +578,16 @@ def run_large_cda(px, ctx, args, e, o, v, P):
 def run_adjoint(args):
     """Full (non-TDA) singlet BSE for `--roots` roots, then the Hellmann-Feynman derivatives of the
     lowest `--adj-roots` with respect to the QP energies (all levels) and the transition block
-    M^P_{vc} (all P, v, c) at fixed screening, checked against central finite differences
-    (h = `--fd-h` Ha) on `--fd-levels` seeded QP levels and `--fd-mmn` seeded Mmn entries; each
-    finite difference re-solves the perturbed problem twice in FP64, started from the converged
-    vectors and converged to `--fd-tol`."""
+    M^P_{vc} (all P, v, c) at fixed screening, checked against finite differences on `--fd-levels`
+    seeded QP levels and `--fd-mmn` seeded Mmn entries (drawn from the 1 % of entries with the
+    largest |dOmega_1/dM|: most of the 1.8e8 entries barely move the low roots, and a difference
+    quotient of a derivative near 1e-9 Ha measures the solver's rounding, not the derivative).
+    Central differences (`--fd-order 2`) or the fourth-order stencil (f(-2h) - 8f(-h) + 8f(h) -
+    f(2h)) / 12h (`--fd-order 4`); each evaluation re-solves the perturbed problem in FP64 from the
+    converged vectors (initial-guess overload) to `--fd-tol`, on the same packed operators (the QP
+    perturbation replaces Hqp, the M perturbation repacks only the transition block).
+    `--fd-offset`/`--fd-count` select a slice of the check list, so one check list can be run in
+    several processes."""
     from votca_amd import pyxtp as px
     o, v, P = args.nocc, args.nvirt, args.naux
     n = o + v
@@ -597,10 +603,13 @@ def run_adjoint(args):
     Mmn.MultiplyRightWithAuxMatrix(U)
     epsinv = np.where(lam > 1e-8, 1.0 / lam, 0.0)
     opt = px.BSEOperator_Options(o - 1, 0, 0, 0, n - 1)
+    H0 = np.diag(e)
+    t0 = time.perf_counter()
+    A = px.SingletOperator_TDA(epsinv, Mmn, H0).configure(opt)
+    B = px.SingletOperator_BTDA_B(epsinv, Mmn, H0).configure(opt)
+    t_pack = time.perf_counter() - t0
 
-    def solve(Hqp, guess=None, tol=None):
-        A = px.SingletOperator_TDA(epsinv, Mmn, Hqp).configure(opt)
-        B = px.SingletOperator_BTDA_B(epsinv, Mmn, Hqp).configure(opt)
+    def solve(guess=None, tol=None, neigen=None):
         ds = px.DavidsonSolver()
         ds.set_tolerance(args.tol if tol is None else tol)
         ds.set_correction("DPR")
@@ -608,17 +617,15 @@ def run_adjoint(args):
         ds.set_max_search_space(args.adj_space or 10 * args.roots)
         ds.set_iter_max(args.iter_max)
         t0 = time.perf_counter()
-        ev, Xv, Yv = ds.solve_full(A, B, args.roots, initial_guess=guess)
+        ev, Xv, Yv = ds.solve_full(A, B, neigen or args.roots, initial_guess=guess)
         wall = time.perf_counter() - t0
-        if guess is not None:
-            log(f"[adjoint] perturbed solve from the converged vectors: {wall:.2f} s, {ds.info.iterations} it")
         if not ds.converged():
             raise RuntimeError(f"full BSE did not converge (residual {ds.info.max_residual:.2e})")
-        return A, B, ev, Xv, Yv, wall, ds.info
+        return ev, Xv, Yv, wall, ds.info
 
     na = args.adj_roots
 
-    def stationary(A, B, Xv, Yv):
+    def stationary(Xv, Yv):
         """Omega from the variational form Omega^2 = [u'(A+B)u][w'(A-B)w] / (u'w)^2, u = X+Y,
         w = X-Y (from (A+B)u = Omega w, (A-B)w = Omega u): second order in the eigenvector error,
         so the difference quotients are not limited by the solver's first-order Ritz values."""
@@ -627,11 +634,11 @@ def run_adjoint(args):
         amb = A.matmul(w) - B.matmul(w)
         return np.sqrt(np.sum(u * apb, 0) * np.sum(w * amb, 0)) / np.abs(np.sum(u * w, 0))
 
-    H0 = np.diag(e)
-    A, B, ev, Xv, Yv, wall, info = solve(H0)
+    ev, Xv, Yv, wall, info = solve()
+    log(f"[adjoint] full BSE: {wall:.2f} s, {info.iterations} it, residual {info.max_residual:.1e} "
+        f"(operators packed in {t_pack:.1f} s)")
     log(f"[adjoint] stationary vs Ritz values of the {na} roots: "
-        f"max |diff| {np.abs(stationary(A, B, Xv, Yv) - ev[:na]).max():.2e} Ha")
-    log(f"[adjoint] full BSE: {wall:.2f} s, {info.iterations} it, residual {info.max_residual:.1e}")
+        f"max |diff| {np.abs(stationary(Xv, Yv) - ev[:na]).max():.2e} Ha")
     N = A.size
     vt, ct = A.vt, A.ct
     d_eps = np.zeros((vt + ct, na), order="F")
@@ -645,60 +652,92 @@ def run_adjoint(args):
                                                  dm.ptr))
     ctx.synchronize()
     t_adj = time.perf_counter() - t0
-    log(f"[adjoint] d/d(e_QP) and d/dM for {na} roots: {t_adj*1e3:.1f} ms ({na*P*N*8/1e9:.2f} GB of dOmega/dM)")
-    ev_stat = stationary(A, B, Xv, Yv)
-    A.close(), B.close()
-    # the perturbed problems (h ~ 1e-5) start from the converged [X; Y] (initial-guess overload)
-    warm = np.vstack([Xv, Yv])
+    # the backward pass's form: one weighted derivative (sum_r g_r dOmega_r) in a P x N array
+    g = np.ones(na) / na
+    dmw = px.DeviceBuffer(ctx, P * N * 8)
+    dew = np.zeros(vt + ct)
+    t0 = time.perf_counter()
+    px._check(px.lib().xtpgpu_bse_adjoint_weighted_device(A.h, B.h, na, px._dptr(Xa), px._dptr(Ya), N, px._dptr(g),
+                                                          px._dptr(dew), dmw.ptr))
+    ctx.synchronize()
+    t_adjw = time.perf_counter() - t0
+    del dmw
+    log(f"[adjoint] d/d(e_QP) and d/dM for {na} roots: {t_adj*1e3:.1f} ms ({na*P*N*8/1e9:.2f} GB of dOmega/dM); "
+        f"weighted (one P x N array): {t_adjw*1e3:.1f} ms")
+    ev_stat = stationary(Xv, Yv)
+    # the check list: seeded QP levels; seeded M entries among the 1 % largest |dOmega_1/dM|
     rng = np.random.default_rng(args.seed + 1)
+    levels = [int(x) for x in rng.choice(vt + ct, args.fd_levels, replace=False)]
+    g0 = np.abs(dm.download((P * N,), order="C"))  # root 0, element (P, v, c) at P*N + v*ct + c
+    top = np.argpartition(g0, -max(1, g0.size // 100))[-max(1, g0.size // 100):]
+    del g0
+    picks = rng.choice(top, args.fd_mmn, replace=False)
+    entries = [(int(q // N), int((q % N) // ct), int(q % ct)) for q in picks]
+    todo = [("e_QP", lv) for lv in levels] + [("M", en) for en in entries]
+    todo = todo[args.fd_offset:args.fd_offset + args.fd_count] if args.fd_count > 0 else todo[args.fd_offset:]
+    warm = np.vstack([Xv, Yv])
     h = args.fd_h
+    steps = (1.0, -1.0) if args.fd_order == 2 else (1.0, -1.0, 2.0, -2.0)
     checks = []
-    for lvl in rng.choice(vt + ct, args.fd_levels, replace=False):
-        w = []
-        for sgn in (1.0, -1.0):
-            Hp = H0.copy()
-            Hp[lvl, lvl] += sgn * h
-            a_, b_, evp, xp_, yp_, *_ = solve(Hp, warm, args.fd_tol)
-            w.append(stationary(a_, b_, xp_, yp_))
-            a_.close(), b_.close()
-        fd = (w[0] - w[1]) / (2 * h)
-        checks.append(("e_QP", int(lvl), fd, d_eps[lvl, :na].copy()))
-    # d/dM^P_{vc}: the adjoint's layout is [root][P][v][c]
-    for _ in range(args.fd_mmn):
-        p_, v_, c_ = int(rng.integers(P)), int(rng.integers(vt)), int(rng.integers(ct))
-        col = Mmn[v_][o + c_, :].copy()          # M^P_{v, c} for all P (slice v, row c)
-        w = []
-        for sgn in (1.0, -1.0):
-            m = col.copy()
-            m[p_] += sgn * h
-            Mmn.set_block(v_, o + c_, m.reshape(P, 1, 1), symmetric=True)
-            a_, b_, evp, xp_, yp_, *_ = solve(H0, warm, args.fd_tol)
-            w.append(stationary(a_, b_, xp_, yp_))
-            a_.close(), b_.close()
-        Mmn.set_block(v_, o + c_, col.reshape(P, 1, 1), symmetric=True)
-        fd = (w[0] - w[1]) / (2 * h)
-        adj = np.array([dm.download((1,), offset=((r * P + p_) * N + v_ * ct + c_) * 8)[0] for r in range(na)])
-        checks.append(("M", (p_, v_, c_), fd, adj))
-    worst = 0.0
-    rows = []
+    t_fd = time.perf_counter()
+    for kind, where in todo:
+        w = {}
+        if kind == "e_QP":
+            for sgn in steps:
+                Hp = H0.copy()
+                Hp[where, where] += sgn * h
+                A.set_hqp(Hp)
+                _, xp_, yp_, wt, inf = solve(warm, args.fd_tol)
+                w[sgn] = stationary(xp_, yp_)
+            A.set_hqp(H0)
+            adj = d_eps[where, :na].copy()
+        else:
+            p_, v_, c_ = where
+            col = Mmn[v_][o + c_, :].copy()          # M^P_{v, c} for all P (slice v, row c)
+            for sgn in steps:
+                m = col.copy()
+                m[p_] += sgn * h
+                Mmn.set_block(v_, o + c_, m.reshape(P, 1, 1), symmetric=True)
+                A.refresh_transitions()
+                B.refresh_transitions()
+                _, xp_, yp_, wt, inf = solve(warm, args.fd_tol)
+                w[sgn] = stationary(xp_, yp_)
+            Mmn.set_block(v_, o + c_, col.reshape(P, 1, 1), symmetric=True)
+            A.refresh_transitions()
+            B.refresh_transitions()
+            adj = np.array([dm.download((1,), offset=((r * P + p_) * N + v_ * ct + c_) * 8)[0] for r in range(na)])
+        if args.fd_order == 2:
+            fd = (w[1.0] - w[-1.0]) / (2 * h)
+        else:
+            fd = (8.0 * (w[1.0] - w[-1.0]) - (w[2.0] - w[-2.0])) / (12 * h)
+        checks.append((kind, where, fd, adj))
+        rel = float(np.abs(fd - adj).max() / max(np.abs(fd).max(), 1e-300))
+        log(f"[adjoint] {kind} {where}: max rel err {rel:.2e} (|d| up to {np.abs(fd).max():.1e}; last solve "
+            f"{wt:.1f} s, {inf.iterations} it)")
+    t_fd = time.perf_counter() - t_fd
     fd_all = np.concatenate([c[2] for c in checks]) if checks else np.zeros(1)
     adj_all = np.concatenate([c[3] for c in checks]) if checks else np.zeros(1)
     normwise = float(np.linalg.norm(adj_all - fd_all) / max(np.linalg.norm(fd_all), 1e-300))
     # rounding floor of a central difference: a few ulps of Omega over 2h
     noise = float(4 * np.finfo(float).eps * np.abs(ev_stat).max() / (2 * h))
-    log(f"[adjoint] norm-wise relative error over all checks {normwise:.2e}; "
-        f"finite-difference rounding floor ~{noise:.1e} (absolute)")
+    rows, worst = [], 0.0
     for kind, where, fd, adj in checks:
-        scale = max(np.abs(fd).max(), 1e-12)
-        rel = float(np.abs(fd - adj).max() / scale)
+        rel = float(np.abs(fd - adj).max() / max(np.abs(fd).max(), 1e-300))
         worst = max(worst, rel)
         rows.append({"param": kind, "index": where, "fd": [float(x) for x in fd], "adjoint": [float(x) for x in adj],
                      "max_rel_err": rel})
-        log(f"[adjoint] {kind} {where}: max rel err {rel:.2e}")
+    log(f"[adjoint] {len(checks)} checks: worst relative error {worst:.2e}, norm-wise {normwise:.2e}; "
+        f"finite-difference rounding floor ~{noise:.1e} (absolute); {t_fd:.0f} s")
     print(json.dumps({"workload": f"full BSE + adjoint, n_occ={o} n_virt={v} n_aux={P}, {args.roots} roots",
-                      "solve_s": wall, "iterations": info.iterations, "adjoint_s": t_adj, "adjoint_roots": na,
-                      "dOmega_dM_bytes": na * P * N * 8, "fd_h": h, "fd_tol": args.fd_tol, "max_rel_err": worst,
-                      "normwise_rel_err": normwise, "fd_rounding_floor": noise, "checks": rows}))
+                      "solve_s": wall, "iterations": info.iterations, "pack_s": t_pack, "adjoint_s": t_adj,
+                      "adjoint_weighted_s": t_adjw, "adjoint_roots": na,
+                      "memory": {"dOmega_dM_all_roots_bytes": na * P * N * 8,
+                                 "dOmega_dM_weighted_bytes": 2 * P * N * 8,
+                                 "operators_resident_bytes": A.memory_bytes() + B.memory_bytes(),
+                                 "mmn_window_bytes": n * n * P * 8},
+                      "fd_h": h, "fd_order": args.fd_order, "fd_tol": args.fd_tol, "fd_offset": args.fd_offset,
+                      "check_list": {"levels": levels, "m_entries": entries}, "max_rel_err": worst,
+                      "normwise_rel_err": normwise, "fd_rounding_floor": noise, "fd_s": t_fd, "checks": rows}))
 
 
 def main():
@@ -731,6 +770,9 @@ def main():
     ap.add_argument("--fd-levels", type=int, default=3)
     ap.add_argument("--fd-mmn", type=int, default=3)
     ap.add_argument("--fd-h", type=float, default=1e-5)
+    ap.add_argument("--fd-order", type=int, default=2, choices=[2, 4])
+    ap.add_argument("--fd-offset", type=int, default=0, help="adjoint workload: first check of the list")
+    ap.add_argument("--fd-count", type=int, default=0, help="adjoint workload: checks to run (0: to the end)")
     ap.add_argument("--fd-tol", type=float, default=1e-9,
                     help="residual tolerance of the perturbed full-BSE solves of the central differences")
     ap.add_argument("--adj-space", type=int, default=0, help="full-BSE max search space (default 10 * roots)")
@@ -747,7 +789,12 @@ def main():
     ap.add_argument("--large-virt", type=int, default=2250)
     ap.add_argument("--large-aux", type=int, default=6000)
     ap.add_argument("--large-roots", type=int, default=50)
-    ap.add_argument("--large-tol", type=float, default=1e-6)
+    ap.add_argument("--large-tol", type=float, default=1e-5)
+    ap.add_argument("--large-sigma0", type=float, default=0.005,
+                    help="large workload: sigma0 of the synthetic law (0.01 at n_aux=6000 gives exchange "
+                         "integrals ~3x those of the single segment, and the 50-root Davidson stalls near 5e-3)")
+    ap.add_argument("--large-space-factor", type=int, default=20,
+                    help="large workload: Davidson max search space in units of roots")
     ap.add_argument("--large-iter", type=int, default=300)
     ap.add_argument("--large-cda-levels", type=int, default=2,
                     help="large workload: Sigma_c^CDA at this many levels on each side of the gap")

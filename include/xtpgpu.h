@@ -178,6 +178,13 @@ int xtpgpu_bse_create_streamed(xtpgpu_ctx ctx, xtpgpu_mmn_chunk_fn source, void*
                                int64_t rpamin, int64_t qpmin, int64_t vmin, int64_t cmax, int cqp, int cx,
                                int cd, int cd2, unsigned precision_mask, int flags, int64_t chunk_bytes,
                                xtpgpu_bseop* out);
+/* Replace the operator's QP Hamiltonian (BSE-sized, as at create) without repacking the tensor
+ * blocks; the diagonal follows. */
+int xtpgpu_bse_set_hqp(xtpgpu_bseop op, const double* Hqp, int64_t ldhqp);
+/* Repack the layouts of the transition block M^P_{vc} (exchange and Hd2 terms) and the diagonal
+ * from `mmn` after its transition block changed (xtpgpu_mmn_set_block); the (v v') and (c c')
+ * blocks and the screening are kept. For operators created from a resident tensor. */
+int xtpgpu_bse_refresh(xtpgpu_bseop op, xtpgpu_mmn mmn);
 /* Bytes of device memory the operator holds (packed layouts and grow-only scratch). */
 int xtpgpu_bse_memory(xtpgpu_bseop op, int64_t* bytes);
 /* BSE::AdjustHqpSize (bse.cc:147-186), host only: the GW-window QP Hamiltonian Hqp (gw x gw,
@@ -447,6 +454,12 @@ int xtpgpu_bse_adjoint(xtpgpu_bseop A, xtpgpu_bseop B, int64_t nroots, const dou
 /* Same with d_m a device buffer (stays resident for device-side consumers such as autograd). */
 int xtpgpu_bse_adjoint_device(xtpgpu_bseop A, xtpgpu_bseop B, int64_t nroots, const double* X,
                               const double* Y, int64_t ldv, double* d_eps, double* d_m_dev);
+/* The derivatives of sum_r weights[r] Omega_r (a backward pass's vector-Jacobian product): d_eps
+ * (nullable, host) vtotal+ctotal values, d_m_dev (nullable, device) auxsize x vtotal x ctotal; one
+ * root's derivative at a time, so the device holds two P x N arrays instead of nroots. */
+int xtpgpu_bse_adjoint_weighted_device(xtpgpu_bseop A, xtpgpu_bseop B, int64_t nroots, const double* X,
+                                       const double* Y, int64_t ldv, const double* weights,
+                                       double* d_eps, double* d_m_dev);
 
 #ifdef __cplusplus
 }

@@ -873,6 +873,41 @@ def test_autograd_function_matches_finite_differences(ctx):
         mm[P, v, c] -= h
         fd = (L(e.detach(), mp) - L(e.detach(), mm)) / (2 * h)
         assert abs(fd - float(m.grad[P, v, c])) < 1e-5 * max(1e-3, abs(fd))
+    # the operators were packed once; new QP energies only replaced Hqp, new transitions repacked
+    # the transition block (two per M entry)
+    assert prob.packs == 1 and prob.refreshes == 4
+    # a backward after another forward: the gradient of its own forward
+    e2 = e.detach().clone().requires_grad_(True)
+    m2 = m.detach().clone().requires_grad_(True)
+    l2 = (excitation_energies(e2, m2, prob) * w).sum()
+    with torch.no_grad():
+        excitation_energies(e.detach(), mp, prob)
+    l2.backward()
+    assert torch.allclose(m2.grad, m.grad, rtol=1e-8, atol=1e-14) and torch.allclose(e2.grad, e.grad, rtol=1e-8)
+
+
+def test_operator_set_hqp_and_refresh_match_fresh_pack(ctx):
+    """xtpgpu_bse_set_hqp and xtpgpu_bse_refresh give the operator a fresh pack would: the same
+    products bit for bit and the same diagonal."""
+    M, eps, Hqp, homo, rpamin, vmin, cmax = random_bse_case(61, vt=5, ct=20, P=30, scale=0.05)
+    vt, ct = homo - vmin + 1, cmax - homo
+    t = dev_mmn(ctx, M)
+    opt = px.BSEOperator_Options(homo, rpamin, 0, vmin, cmax)
+    rng = np.random.default_rng(8)
+    Xm = rng.normal(size=(vt * ct, 3))
+    for c in ((1, 2, 1, 0), (0, 2, 0, 1), (1, 0, 1, 0)):
+        op = px.BSE_OPERATOR(*c, eps, t, Hqp, precisions=("fp64", "bf16x3")).configure(opt)
+        H2 = Hqp + np.diag(rng.uniform(-0.01, 0.01, Hqp.shape[0]))
+        op.set_hqp(H2)
+        m_new = 0.9 * np.transpose(M[vmin - rpamin:homo + 1 - rpamin, homo + 1 - rpamin:cmax + 1 - rpamin, :],
+                                   (2, 0, 1))
+        t.set_block(vmin - rpamin, homo + 1 - rpamin, m_new, symmetric=True)
+        op.refresh_transitions()
+        ref = px.BSE_OPERATOR(*c, eps, t, H2, precisions=("fp64", "bf16x3")).configure(opt)
+        for prec in ("fp64", "bf16x3"):
+            assert np.array_equal(op.matmul(Xm, prec), ref.matmul(Xm, prec)), (c, prec)
+        assert np.array_equal(op.diagonal(), ref.diagonal()), c
+        t.set_tensor(M)
 
 
 CDA_E = np.array([0.0468207, 0.0907801, 0.0907801, 0.104563, 0.592491, 0.663355, 0.663355, 0.768373, 1.69292,

@@ -956,7 +956,8 @@ struct xtpgpu_bseop_s {
   void* src_user = nullptr;
   bool stream64 = false;
   bool screen_full = false;  // Bvv carries the full eps^-1(0) matrix (unrotated aux basis)
-  long cmin_r = 0;
+  long cmin_r = 0, vmin_r = 0;  // tensor indices of the first conduction / valence level
+  DevBuf diag0;                 // the diagonal without its Hqp part (xtpgpu_bse_set_hqp)
   DevBuf srcbuf, mcc64;
 };
 
@@ -1321,9 +1322,9 @@ namespace {
 
 // The BSE-sized Hqp blocks (bse.cc:147-186) and the screening weights eps0^-1 of an operator whose
 // vt, ct, P and cqp are set.
-void setup_hqp_eps(xtpgpu_bseop op, const double* Hqp, int64_t ldhqp, const double* eps0inv) {
+void setup_hqp(xtpgpu_bseop op, const double* Hqp, int64_t ldhqp) {
   hipStream_t st = op->ctx->stream;
-  const long vt = op->vt, ct = op->ct, P = op->P, hs = vt + ct;
+  const long vt = op->vt, ct = op->ct, hs = vt + ct;
   if (op->cqp) {
     XG_REQUIRE(ldhqp >= hs, XTPGPU_ERR_SHAPE, "xtpgpu_bse_create: ldhqp < vtotal+ctotal");
     std::vector<double> hc(ct * ct), hv(vt * vt), hcd(ct), hvd(vt);
@@ -1346,12 +1347,121 @@ void setup_hqp_eps(xtpgpu_bseop op, const double* Hqp, int64_t ldhqp, const doub
     XG_HIP(hipMemcpyAsync(op->hvd.p, hvd.data(), vt * 8, hipMemcpyHostToDevice, st));
     XG_HIP(hipStreamSynchronize(st));
   }
+}
+
+// diag = diag0 + cqp (Hqp_cc - Hqp_vv)
+void finish_diag(xtpgpu_bseop op) {
+  const long N = op->N, ct = op->ct;
+  double* dg = op->diag.as<double>();
+  const double* d0 = op->diag0.as<double>();
+  const double* hcd = op->cqp ? op->hcd.as<double>() : nullptr;
+  const double* hvd = op->cqp ? op->hvd.as<double>() : nullptr;
+  const double fqp = op->cqp;
+  launch_map(op->ctx->stream, (size_t)N, [=] __device__(size_t i) {
+    const long v = (long)i / ct, c = (long)i % ct;
+    dg[i] = d0[i] + (fqp != 0.0 ? fqp * (hcd[c] - hvd[v]) : 0.0);
+  });
+}
+
+void setup_hqp_eps(xtpgpu_bseop op, const double* Hqp, int64_t ldhqp, const double* eps0inv) {
+  hipStream_t st = op->ctx->stream;
+  const long P = op->P;
+  setup_hqp(op, Hqp, ldhqp);
   op->eps.alloc(P * 8);
   if (eps0inv) {
     XG_HIP(hipMemcpyAsync(op->eps.p, eps0inv, P * 8, hipMemcpyHostToDevice, st));
   } else {
     fill_zero(st, op->eps.as<double>(), P);
   }
+}
+
+}  // namespace
+
+namespace {
+
+// The layouts that hold the transition block M^P_{vc} (Mvc, MvcT, McvT and their BF16 shadows),
+// packed from a resident tensor; xtpgpu_bse_refresh repacks them after the tensor's transition
+// block changed.
+void pack_transitions(xtpgpu_bseop op, xtpgpu_mmn mmn) {
+  hipStream_t st = op->ctx->stream;
+  const long vt = op->vt, ct = op->ct, P = op->P, N = op->N, ctp = op->ctp, vtp = op->vtp, Pp = op->Pp;
+  const long Np = vt * ctp, nt = mmn->ntotal, ps = (long)mmn->pstride();
+  const long vmin_r = op->vmin_r, cmin_r = op->cmin_r;
+  const int cx = op->cx, cd2 = op->cd2;
+  const bool need16 = ((op->mask >> XTPGPU_BF16) & 1u) || ((op->mask >> XTPGPU_BF16X3) & 1u);
+  const double* D = mmn->d();
+  auto el = [=] __device__(long p, long slice, long row) { return D[p * ps + slice * nt + row]; };
+  if (cx || cd2) {
+    op->Mvc.ensure((size_t)P * Np * 8);
+    double* A = op->Mvc.as<double>();
+    launch_map(st, (size_t)P * Np, [=] __device__(size_t i) {
+      long p = (long)(i / (size_t)Np), r = (long)(i % (size_t)Np);
+      long v = r / ctp, c = r % ctp;
+      A[i] = c < ct ? el(p, vmin_r + v, cmin_r + c) : 0.0;
+    });
+    if (need16 && cd2) {
+      op->Mvc_h.ensure((size_t)P * Np * 2); op->Mvc_l.ensure((size_t)P * Np * 2);
+      split_to_bf16(st, A, op->Mvc_h.as<unsigned short>(), op->Mvc_l.as<unsigned short>(), (size_t)P * Np);
+    }
+  }
+  if (cx) {
+    op->MvcT.ensure((size_t)N * Pp * 8);
+    double* A = op->MvcT.as<double>();
+    launch_map(st, (size_t)N * Pp, [=] __device__(size_t i) {
+      long r = (long)(i / (size_t)Pp), p = (long)(i % (size_t)Pp);
+      long v = r / ct, c = r % ct;
+      A[i] = p < P ? el(p, vmin_r + v, cmin_r + c) : 0.0;
+    });
+  }
+  if (cd2) {
+    op->McvT.ensure((size_t)ct * P * vtp * 8);
+    double* A = op->McvT.as<double>();
+    launch_map(st, (size_t)ct * P * vtp, [=] __device__(size_t i) {
+      long c1 = (long)(i / (size_t)(P * vtp)), r = (long)(i % (size_t)(P * vtp));
+      long p = r / vtp, v2 = r % vtp;
+      A[i] = v2 < vt ? el(p, cmin_r + c1, vmin_r + v2) : 0.0;
+    });
+    if (need16) {
+      op->McvT_h.ensure((size_t)ct * P * vtp * 2); op->McvT_l.ensure((size_t)ct * P * vtp * 2);
+      split_to_bf16(st, A, op->McvT_h.as<unsigned short>(), op->McvT_l.as<unsigned short>(), (size_t)ct * P * vtp);
+    }
+  }
+}
+
+// BSE_OPERATOR::diagonal (bse_operator.cc:134-175) from a resident tensor: diag0 (exchange and
+// direct terms), then the Hqp part.
+void build_diag(xtpgpu_bseop op, xtpgpu_mmn mmn) {
+  hipStream_t st = op->ctx->stream;
+  const long ct = op->ct, P = op->P, N = op->N, nt = mmn->ntotal, ps = (long)mmn->pstride();
+  const long vmin_r = op->vmin_r, cmin_r = op->cmin_r;
+  const double* D = mmn->d();
+  const double* E = op->eps.as<double>();
+  auto el = [=] __device__(long p, long slice, long row) { return D[p * ps + slice * nt + row]; };
+  op->diag.ensure((size_t)N * 8);
+  op->diag0.ensure((size_t)N * 8);
+  double* d0 = op->diag0.as<double>();
+  const double fx = op->cx, fd = op->cd, fd2 = op->cd2;
+  launch_map(st, (size_t)N, [=] __device__(size_t i) {
+    long v = (long)i / ct, c = (long)i % ct;
+    double e = 0.0;
+    if (fx != 0.0) {
+      double s = 0.0;
+      for (long p = 0; p < P; ++p) { double m = el(p, vmin_r + v, cmin_r + c); s += m * m; }
+      e += fx * s;
+    }
+    if (fd != 0.0) {
+      double s = 0.0;
+      for (long p = 0; p < P; ++p) s += el(p, cmin_r + c, cmin_r + c) * E[p] * el(p, vmin_r + v, vmin_r + v);
+      e -= fd * s;
+    }
+    if (fd2 != 0.0) {
+      double s = 0.0;
+      for (long p = 0; p < P; ++p) s += el(p, cmin_r + c, vmin_r + v) * E[p] * el(p, vmin_r + v, cmin_r + c);
+      e -= fd2 * s;
+    }
+    d0[i] = e;
+  });
+  finish_diag(op);
 }
 
 }  // namespace
@@ -1413,70 +1523,10 @@ extern "C" int xtpgpu_bse_create(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* e
         split_to_bf16(st, B, op->Bvv_h.as<unsigned short>(), op->Bvv_l.as<unsigned short>(), (size_t)vt * P * vtp);
       }
     }
-    if (cx || cd2) {
-      op->Mvc.alloc((size_t)P * Np * 8);
-      double* A = op->Mvc.as<double>();
-      launch_map(st, (size_t)P * Np, [=] __device__(size_t i) {
-        long p = (long)(i / (size_t)Np), r = (long)(i % (size_t)Np);
-        long v = r / ctp, c = r % ctp;
-        A[i] = c < ct ? el(p, vmin_r + v, cmin_r + c) : 0.0;
-      });
-      if (need16 && cd2) {
-        op->Mvc_h.alloc((size_t)P * Np * 2); op->Mvc_l.alloc((size_t)P * Np * 2);
-        split_to_bf16(st, A, op->Mvc_h.as<unsigned short>(), op->Mvc_l.as<unsigned short>(), (size_t)P * Np);
-      }
-    }
-    if (cx) {
-      op->MvcT.alloc((size_t)N * Pp * 8);
-      double* A = op->MvcT.as<double>();
-      launch_map(st, (size_t)N * Pp, [=] __device__(size_t i) {
-        long r = (long)(i / (size_t)Pp), p = (long)(i % (size_t)Pp);
-        long v = r / ct, c = r % ct;
-        A[i] = p < P ? el(p, vmin_r + v, cmin_r + c) : 0.0;
-      });
-    }
-    if (cd2) {
-      op->McvT.alloc((size_t)ct * P * vtp * 8);
-      double* A = op->McvT.as<double>();
-      launch_map(st, (size_t)ct * P * vtp, [=] __device__(size_t i) {
-        long c1 = (long)(i / (size_t)(P * vtp)), r = (long)(i % (size_t)(P * vtp));
-        long p = r / vtp, v2 = r % vtp;
-        A[i] = v2 < vt ? el(p, cmin_r + c1, vmin_r + v2) : 0.0;
-      });
-      if (need16) {
-        op->McvT_h.alloc((size_t)ct * P * vtp * 2); op->McvT_l.alloc((size_t)ct * P * vtp * 2);
-        split_to_bf16(st, A, op->McvT_h.as<unsigned short>(), op->McvT_l.as<unsigned short>(), (size_t)ct * P * vtp);
-      }
-    }
-    // BSE_OPERATOR::diagonal (bse_operator.cc:134-175)
-    op->diag.alloc((size_t)N * 8);
-    {
-      double* dg = op->diag.as<double>();
-      const double* hcd = cqp ? op->hcd.as<double>() : nullptr;
-      const double* hvd = cqp ? op->hvd.as<double>() : nullptr;
-      const double fx = cx, fqp = cqp, fd = cd, fd2 = cd2;
-      launch_map(st, (size_t)N, [=] __device__(size_t i) {
-        long v = (long)i / ct, c = (long)i % ct;
-        double e = 0.0;
-        if (fx != 0.0) {
-          double s = 0.0;
-          for (long p = 0; p < P; ++p) { double m = el(p, vmin_r + v, cmin_r + c); s += m * m; }
-          e += fx * s;
-        }
-        if (fqp != 0.0) e += fqp * (hcd[c] - hvd[v]);
-        if (fd != 0.0) {
-          double s = 0.0;
-          for (long p = 0; p < P; ++p) s += el(p, cmin_r + c, cmin_r + c) * E[p] * el(p, vmin_r + v, vmin_r + v);
-          e -= fd * s;
-        }
-        if (fd2 != 0.0) {
-          double s = 0.0;
-          for (long p = 0; p < P; ++p) s += el(p, cmin_r + c, vmin_r + v) * E[p] * el(p, vmin_r + v, cmin_r + c);
-          e -= fd2 * s;
-        }
-        dg[i] = e;
-      });
-    }
+    op->vmin_r = vmin_r;
+    op->cmin_r = cmin_r;
+    pack_transitions(op.get(), mmn);
+    build_diag(op.get(), mmn);
     XG_HIP(hipStreamSynchronize(st));
     *out = op.release();
   });
@@ -1694,17 +1744,39 @@ extern "C" int xtpgpu_bse_create_streamed(xtpgpu_ctx ctx, xtpgpu_mmn_chunk_fn so
       split_to_bf16(st, Mvc, op->Mvc_h.as<unsigned short>(), op->Mvc_l.as<unsigned short>(), (size_t)P * Np);
       split_to_bf16(st, McvT, op->McvT_h.as<unsigned short>(), op->McvT_l.as<unsigned short>(), (size_t)ct * P * vtp);
     }
-    if (cqp) {
-      const double* hcd = op->hcd.as<double>();
-      const double* hvd = op->hvd.as<double>();
-      const double fqp = cqp;
-      launch_map(st, (size_t)N, [=] __device__(size_t i) {
-        const long v = (long)i / ct, c = (long)i % ct;
-        dg[i] += fqp * (hcd[c] - hvd[v]);
-      });
-    }
+    op->diag0.alloc((size_t)N * 8);
+    XG_HIP(hipMemcpyAsync(op->diag0.p, dg, (size_t)N * 8, hipMemcpyDeviceToDevice, st));
+    finish_diag(op.get());
     XG_HIP(hipStreamSynchronize(st));
     *out = op.release();
+  });
+}
+
+extern "C" int xtpgpu_bse_set_hqp(xtpgpu_bseop op, const double* Hqp, int64_t ldhqp) {
+  return guarded([&] {
+    XG_REQUIRE(op && (Hqp || op->cqp == 0), XTPGPU_ERR_ARG, "xtpgpu_bse_set_hqp: null argument");
+    XG_HIP(hipSetDevice(op->ctx->device));
+    hipStream_t st = op->ctx->stream;
+    XG_HIP(hipStreamSynchronize(st));  // no product may still read the old blocks
+    setup_hqp(op, Hqp, ldhqp);
+    finish_diag(op);
+    XG_HIP(hipStreamSynchronize(st));
+  });
+}
+
+extern "C" int xtpgpu_bse_refresh(xtpgpu_bseop op, xtpgpu_mmn mmn) {
+  return guarded([&] {
+    XG_REQUIRE(op && mmn, XTPGPU_ERR_ARG, "xtpgpu_bse_refresh: null argument");
+    XG_REQUIRE(!op->stream64 && !op->screen_full, XTPGPU_ERR_ARG,
+               "xtpgpu_bse_refresh: operator was packed from a chunk source");
+    XG_REQUIRE(mmn->aux == op->P && op->cmin_r + op->ct <= mmn->mtotal && op->cmin_r + op->ct <= mmn->ntotal,
+               XTPGPU_ERR_SHAPE, "xtpgpu_bse_refresh: tensor does not match the operator");
+    XG_HIP(hipSetDevice(op->ctx->device));
+    hipStream_t st = op->ctx->stream;
+    XG_HIP(hipStreamSynchronize(st));
+    pack_transitions(op, mmn);
+    build_diag(op, mmn);
+    XG_HIP(hipStreamSynchronize(st));
   });
 }
 

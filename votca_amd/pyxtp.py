@@ -230,6 +230,9 @@ SIGNATURES = {
                                                _dp, _i64, C.POINTER(DavidsonInfo)]),
     "xtpgpu_bse_adjoint": (C.c_int, [_vp, _vp, _i64, _dp, _dp, _i64, _dp, _dp]),
     "xtpgpu_bse_adjoint_device": (C.c_int, [_vp, _vp, _i64, _dp, _dp, _i64, _dp, _vp]),
+    "xtpgpu_bse_adjoint_weighted_device": (C.c_int, [_vp, _vp, _i64, _dp, _dp, _i64, _dp, _dp, _vp]),
+    "xtpgpu_bse_set_hqp": (C.c_int, [_vp, _dp, _i64]),
+    "xtpgpu_bse_refresh": (C.c_int, [_vp, _vp]),
     "xtpgpu_sigma_cda_create": (C.c_int, [_vp, _vp, _dp, _i64, _i64, _i64, _i64, _i64, C.c_double, C.c_double,
                                           _dp, _dp, _i64, C.c_int, C.POINTER(_vp)]),
     "xtpgpu_sigma_cda_destroy": (C.c_int, [_vp]),
@@ -666,6 +669,15 @@ class BSE_OPERATOR:
         self.vt = opt.homo - opt.vmin + 1
         self.ct = opt.cmax - opt.homo
         return self
+
+    def set_hqp(self, Hqp):
+        """Replace the QP Hamiltonian (BSE-sized) without repacking the tensor blocks."""
+        self.Hqp = _f64(Hqp)
+        _check(lib().xtpgpu_bse_set_hqp(self.h, _dptr(self.Hqp), self.Hqp.shape[0]))
+
+    def refresh_transitions(self):
+        """Repack the transition-block layouts and the diagonal after Mmn's (v, c) block changed."""
+        _check(lib().xtpgpu_bse_refresh(self.h, self.Mmn.h))
 
     def memory_bytes(self) -> int:
         n = _i64(0)
