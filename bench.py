@@ -36,12 +36,23 @@ PEAK = {"fp64": 78.6e12, "bf16": 2.5e15, "hbm": 8.0e12}
 # from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of
 # `bench.py --workload matvec --steps 1 [--mixed bf16] --copy-calibration`, each scaled by what the
 # same counter reads for the 1 GiB copy16_kernel in that run (FETCH_SIZE 0.5 GiB -> x2.0,
-# WRITE_SIZE 1.0 GiB -> x1.0). The P-chunk keeps the stage-1 intermediate under 3 GiB: 286 levels
-# for the split form (two output planes), 573 for plain BF16.
-HD_STAGE1_TRAFFIC = {"bf16x3": {"read": 4.078 * 2.0 * 2**30, "write": 3.311 * 1.0 * 2**30,
+# WRITE_SIZE 1.0 GiB -> x1.0); FETCH_SIZE 4.462e6 / 3.159e6 KiB, WRITE_SIZE 3.450e6 / 3.490e6 KiB
+# per dispatch. The P-chunk keeps the stage-1 intermediate under 3 GiB: 286 levels for the split
+# form (two output planes), 573 for plain BF16.
+HD_STAGE1_TRAFFIC = {"bf16x3": {"read": 4.462e6 * 2.0 * 1024, "write": 3.450e6 * 1.0 * 1024,
                                 "launch": "30-column H X stage-1 launch, split BF16 (P-chunk 286)"},
-                     "bf16": {"read": 1.958 * 2.0 * 2**30, "write": 3.074 * 1.0 * 2**30,
+                     "bf16": {"read": 3.159e6 * 2.0 * 1024, "write": 3.490e6 * 1.0 * 1024,
                               "launch": "30-column H X stage-1 launch, plain BF16 (P-chunk 573)"}}
+
+
+# The same launches under rocprofv3 SQ/TCP counters (profiles/hd_stage1_100x900x2000_counters.md):
+# the clock the chip holds under the load (GRBM_GUI_ACTIVE / 8 XCDs / duration), MFMA busy
+# (SQ_VALU_MFMA_BUSY_CYCLES / cycles / 1024 SIMDs) and the share of cycles the L1 waits on L2 data
+# (TCP_PENDING_STALL_CYCLES_sum / 256 CUs / cycles).
+HD_STAGE1_COUNTERS = {"bf16x3": {"clock_GHz": 1.95, "peak_at_clock_tflops": 2500.0 * 1.95 / 2.4,
+                                 "mfma_busy": 0.446, "tcp_pending_stall": 0.330},
+                      "bf16": {"clock_GHz": 1.81, "peak_at_clock_tflops": 2500.0 * 1.81 / 2.4,
+                               "mfma_busy": 0.373, "tcp_pending_stall": 0.365}}
 
 
 def log(*a):
@@ -687,7 +698,7 @@ def run_adjoint(args):
                 Hp = H0.copy()
                 Hp[where, where] += sgn * h
                 A.set_hqp(Hp)
-                _, xp_, yp_, wt, inf = solve(warm, args.fd_tol)
+                _, xp_, yp_, wt, inf = solve(warm, args.fd_tol, neigen=na)
                 w[sgn] = stationary(xp_, yp_)
             A.set_hqp(H0)
             adj = d_eps[where, :na].copy()
@@ -700,7 +711,7 @@ def run_adjoint(args):
                 Mmn.set_block(v_, o + c_, m.reshape(P, 1, 1), symmetric=True)
                 A.refresh_transitions()
                 B.refresh_transitions()
-                _, xp_, yp_, wt, inf = solve(warm, args.fd_tol)
+                _, xp_, yp_, wt, inf = solve(warm, args.fd_tol, neigen=na)
                 w[sgn] = stationary(xp_, yp_)
             Mmn.set_block(v_, o + c_, col.reshape(P, 1, 1), symmetric=True)
             A.refresh_transitions()
@@ -969,6 +980,8 @@ def main():
                 "traffic_launch": HD_STAGE1_TRAFFIC[dom]["launch"] if traffic else None,
                 "avg_launch_ms": kt * 1e3, "algorithmic_flops_per_launch": tm["flops"] / max(1, tm["launches"]),
                 "mfma_issued_tflops": issue * ach / 1e12, "mfma_issued_frac": issue * ach / PEAK["bf16"]}
+    if dom in HD_STAGE1_COUNTERS:
+        roofline["counters"] = HD_STAGE1_COUNTERS[dom]
     for other, t in stage1.items():
         if other != dom:
             a2 = t["flops"] / max(t["ms"] / 1e3, 1e-30)
