@@ -543,16 +543,21 @@ def run_large(args):
 
 
 def run_large_cda(px, ctx, args, e, o, v, P):
-    """Sigma_c^CDA (20 Legendre points, sigma_cda.cc) at the intercepts of `--large-cda-levels` levels
-    on each side of the gap, from the tensor rows the CDA reads (levels 0 .. LUMO + levels - 1 over
-    all n, FP64) and the mixed residue screening; the number of residue eps builds that level set and
-    the 100-level window take, and the time per build."""
-    from votca_amd import quadrature
+    """Sigma_c^CDA (20 Legendre points, sigma_cda.cc) for `--large-cda-levels` levels on each side of
+    the gap, from the tensor rows the CDA reads (levels 0 .. LUMO + levels - 1 over all n, FP64),
+    drawn with sigma0 = `--large-gw-sigma0` (at the BSE's 0.005 the synthetic law puts Sigma_c near
+    -20 Ha: 2500 states x 6000 aux functions of uniform size couple far more than real orbitals do),
+    mixed-precision screening: Sigma_c at the intercepts, then GW::SolveQP on those levels with the
+    fixed-point solver (V_xc = diag(Sigma_x), so the intercepts are the input energies); the residue
+    eps builds per evaluation sweep of this level set and of the 100-level window, and the time per
+    build. The grid solver scans +-0.75 Ha around each level, where almost every state between the
+    scanned frequency and the Fermi level adds a residue build: 164 s for these four levels at
+    n_aux = 1500, and ~15x that at 6000."""
     n = o + v
     L = args.large_cda_levels
     qpmin, qpmax = o - L, o + L - 1
     t = px.TCMatrix_gwbse(ctx).Initialize(P, 0, qpmax, 0, n - 1)
-    t.fill_synthetic(args.seed, args.large_sigma0, 3.0)
+    t.fill_synthetic(args.seed + 2, args.large_gw_sigma0, 3.0)
     rpa = px.RPA(t)
     rpa.configure(o - 1, 0, n - 1)
     rpa.setRPAInputEnergies(e)
@@ -562,9 +567,9 @@ def run_large_cda(px, ctx, args, e, o, v, P):
     cda.PrepareScreening()
     prep = time.perf_counter() - t0
     levels = list(range(2 * L))
-    om = [float(e[qpmin + l]) for l in levels]
+    icpt = np.ascontiguousarray(e[qpmin:qpmax + 1])
     t0 = time.perf_counter()
-    sig = cda.diag_batch(levels, om)
+    sig = cda.diag_batch(levels, [float(x) for x in icpt])
     ev = time.perf_counter() - t0
 
     def residues(lo, hi):  # states with a residue at w = e_l (CalcResiduePrefactor, sigma_cda.cc:63-77)
@@ -577,11 +582,18 @@ def run_large_cda(px, ctx, args, e, o, v, P):
 
     nres = residues(qpmin, qpmax)
     n100 = residues(o - 50, o + 49)
-    out = {"levels": [qpmin, qpmax], "prepare_s": prep, "eval_s": ev, "residue_builds": nres,
-           "s_per_residue_build": ev / max(nres, 1), "residue_builds_100_level_window": n100,
-           "sigma_c_Ha": [float(x) for x in sig], "screening": args.mixed}
+    t0 = time.perf_counter()
+    qp, conv = cda.SolveQP(icpt, icpt.copy(), qp_solver="fixedpoint")
+    t_qp = time.perf_counter() - t0
+    out = {"levels": [qpmin, qpmax], "sigma0": args.large_gw_sigma0, "prepare_s": prep, "eval_s": ev,
+           "residue_builds": nres, "s_per_residue_build": ev / max(nres, 1),
+           "residue_builds_100_level_window": n100, "sigma_c_at_intercepts_Ha": [float(x) for x in sig],
+           "qp_solve_s": t_qp, "qp_energies_Ha": [float(x) for x in qp], "qp_converged": [bool(x) for x in conv],
+           "qp_shift_Ha": [float(a - b) for a, b in zip(qp, icpt)], "screening": args.mixed}
     log(f"[large] Sigma_c^CDA at {2*L} intercepts: prepare {prep:.1f} s, evaluation {ev:.1f} s over {nres} residue "
         f"builds; the 100-level window needs {n100} builds per sweep")
+    log(f"[large] CDA QP (fixed point) on levels {qpmin}-{qpmax}: {t_qp:.1f} s, converged {int(np.sum(conv))}/{2*L}, "
+        f"shifts {', '.join(f'{a - b:+.4f}' for a, b in zip(qp, icpt))} Ha")
     cda.close()
     return out
 
@@ -807,6 +819,8 @@ def main():
     ap.add_argument("--large-space-factor", type=int, default=20,
                     help="large workload: Davidson max search space in units of roots")
     ap.add_argument("--large-iter", type=int, default=300)
+    ap.add_argument("--large-gw-sigma0", type=float, default=0.0005,
+                    help="large workload: sigma0 of the tensor the CDA step reads")
     ap.add_argument("--large-cda-levels", type=int, default=2,
                     help="large workload: Sigma_c^CDA at this many levels on each side of the gap")
     args = ap.parse_args()

@@ -134,6 +134,30 @@ def test_rpa_mixed_frequency_groups(ctx, monkeypatch, prec, tol, kind, freqs):
     assert np.array_equal(outs[0], outs[1])  # grouping does not change a single bit
 
 
+@pytest.mark.parametrize("budget", ["1", None])
+def test_rpa_fp64_scaled_copies(ctx, monkeypatch, budget):
+    """FP64 eps with the weights in scaled copies (the default), in groups of one frequency and in
+    one group, against the oracle (1e-13) and the load-weighted FP64 kernel (XTPGPU_EPS64_SCALED=0)."""
+    rng = np.random.default_rng(6)
+    o, v, P = 8, 60, 130
+    M = rng.normal(scale=0.3, size=(o + v, o + v, P))
+    e = np.concatenate([np.sort(rng.uniform(-1, -0.2, o)), np.sort(rng.uniform(0.3, 1.5, v))])
+    rpa = px.RPA(dev_mmn(ctx, M))
+    rpa.configure(o - 1, 0, o + v - 1)
+    rpa.setRPAInputEnergies(e)
+    freqs = [0.0, 0.3, 1.7]
+    monkeypatch.setenv("XTPGPU_EPS64_SCALED", "0")
+    g0 = rpa.calculate_epsilon(px.EPS_IMAG, freqs, "fp64")
+    monkeypatch.setenv("XTPGPU_EPS64_SCALED", "1")
+    if budget:
+        monkeypatch.setenv("XTPGPU_EPS_SCALE_BYTES", budget)
+    g = rpa.calculate_epsilon(px.EPS_IMAG, freqs, "fp64")
+    for i, f in enumerate(freqs):
+        assert relerr(g[:, :, i], X.rpa_epsilon(M, e, o - 1, 0, o + v - 1, f, "imag")) < 1e-13
+        assert relerr(g[:, :, i], g0[:, :, i]) < 1e-14
+        assert np.array_equal(g[:, :, i], g[:, :, i].T)
+
+
 @pytest.mark.parametrize("kern", ["staged32", "staged64"])
 def test_rpa_bf16x3_kernel_variants(ctx, kern, monkeypatch):
     """eps(i w) through both register-staged BF16 kernels (K step 32 and 64, on the scaled
