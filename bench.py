@@ -222,18 +222,8 @@ def run_segments(args):
         rpas.append(r)
     prec = args.mixed if args.segments_precision == "mixed" else "fp64"
     batch = px.EpsBatch(rpas, freqs, prec)
-    # autotune the persistent grid (workgroups per CU) on this batch
-    best = None
-    for mult in (1, 2, 4, 8):
-        batch.set_grid(256 * mult)
-        batch.run(True)
-        t0 = time.perf_counter()
-        batch.run(True)
-        dt = time.perf_counter() - t0
-        log(f"[segments] autotune grid={256*mult}: {dt*1e3:.1f} ms")
-        if best is None or dt < best[1]:
-            best = (mult, dt)
-    batch.set_grid(256 * best[0])
+    grid = batch.autotune()  # persistent grid (workgroups per CU), timed in the library
+    log(f"[segments] autotuned grid: {grid}")
     batch.run(True)
     ts = []
     for _ in range(max(1, args.steps)):
@@ -259,69 +249,86 @@ def run_segments(args):
     res = {"eps_batch_s": t_batch, "eps_serial_s": t_serial, "eps_batch_segments_per_s": nseg / t_batch,
            "eps_serial_segments_per_s": nseg / t_serial, "eps_batch_tflops": F / t_batch / 1e12,
            "eps_serial_latency_ms_p50": float(np.percentile(lat, 50) * 1e3),
-           "eps_serial_latency_ms_p99": float(np.percentile(lat, 99) * 1e3), "autotuned_grid": 256 * best[0],
+           "eps_serial_latency_ms_p99": float(np.percentile(lat, 99) * 1e3), "autotuned_grid": grid,
            "precision": prec}
     log(f"[segments] eps batch {t_batch*1e3:.1f} ms ({nseg/t_batch:.1f} seg/s) vs serial {t_serial*1e3:.1f} ms")
     del batch, rpas, bufs
-    # BSE-TDA, 8 roots per segment (full tensors): one segment at a time, then the same segments
-    # spread over `--bse-streams` host threads, each with its own context (stream, BLAS handle)
+    # BSE-TDA, 8 roots per segment: the operators of the first `--bse-segments` segments are set up
+    # (screening on the device, operator packed; the tensor freed once packed), then solved one after
+    # another with the single-segment entry point and all together with the batched entry point
+    # (`--bse-workers` worker streams), in FP64 and in the mixed form with FP64 refinement to the
+    # same 1e-5 residual. Latency: each segment's completion time from the start of the loop or
+    # batch (all segments submitted at once), and for the loop each segment's own solve time.
     nb = min(nseg, args.bse_segments)
-
-    def bse_segment(c, s):
+    mixed = args.mixed
+    t_prep = time.perf_counter()
+    ops = []
+    for s in range(nb):
         o, v, P = segs[s]
-        t0 = time.perf_counter()
-        t = px.TCMatrix_gwbse(c).Initialize(P, 0, o + v - 1, 0, o + v - 1)
+        t = px.TCMatrix_gwbse(ctx).Initialize(P, 0, o + v - 1, 0, o + v - 1)
         t.fill_synthetic(args.seed * 100003 + s, args.sigma0, 3.0)
         e = synthetic_energies(args.seed * 100003 + s, o, v)
         ei = px.setup_screening(t, e, o - 1, 0, o + v - 1)
-        precs = ("fp64",) if prec == "fp64" else ("fp64", prec)
-        op = px.SingletOperator_TDA(ei, t, np.diag(e), precisions=precs).configure(
-            px.BSEOperator_Options(o - 1, 0, 0, 0, o + v - 1))
+        ops.append(px.SingletOperator_TDA(ei, t, np.diag(e), precisions=("fp64", mixed)).configure(
+            px.BSEOperator_Options(o - 1, 0, 0, 0, o + v - 1)))
+        t.close()
+    ctx.synchronize()
+    t_prep = time.perf_counter() - t_prep
+    res.update({"bse_segments": nb, "bse_setup_s": t_prep, "bse_workers": args.bse_workers})
+    log(f"[segments] BSE set-up of {nb} segments (screening + packing): {t_prep:.2f} s")
+
+    def solver(p):
         ds = px.DavidsonSolver()
         ds.set_tolerance(1e-5)
         ds.set_iter_max(100)
         ds.set_max_search_space(80)
-        if prec != "fp64":  # mixed iterations, FP64 refinement to the same 1e-5 residual
-            ds.set_precision(prec, refine_tolerance=1e-5, refine_max=8)
-        ev, _ = ds.solve(op, 8, want_vectors=False)
+        if p != "fp64":
+            ds.set_precision(p, refine_tolerance=1e-5, refine_max=8)
+        return ds
+
+    def pct(a, q):
+        return float(np.percentile(a, q) * 1e3)
+
+    for p in ("fp64", mixed):
+        tag = "fp64" if p == "fp64" else "mixed"
+        ds = solver(p)
+        ds.solve(ops[0], 8, want_vectors=False)  # first-use allocations outside the timed loops
+        ev_ser, own, done = [], [], []
+        t0 = time.perf_counter()
+        for op in ops:
+            t1 = time.perf_counter()
+            ev, _ = ds.solve(op, 8, want_vectors=False)
+            t2 = time.perf_counter()
+            if not ds.converged():
+                raise RuntimeError(f"segment BSE ({p}) did not converge")
+            ev_ser.append(ev.copy())
+            own.append(t2 - t1)
+            done.append(t2 - t0)
+        t_ser = time.perf_counter() - t0
+        res.update({f"bse_{tag}_serial_s": t_ser, f"bse_{tag}_serial_segments_per_s": nb / t_ser,
+                    f"bse_{tag}_serial_completion_ms_p50": pct(done, 50),
+                    f"bse_{tag}_serial_completion_ms_p99": pct(done, 99),
+                    f"bse_{tag}_serial_solve_ms_p50": pct(own, 50), f"bse_{tag}_serial_solve_ms_p99": pct(own, 99)})
+        log(f"[segments] BSE-TDA {p}: serial {t_ser:.2f} s ({nb / t_ser:.1f} seg/s, completion p50/p99 "
+            f"{pct(done, 50):.0f}/{pct(done, 99):.0f} ms, own solve p50/p99 {pct(own, 50):.0f}/{pct(own, 99):.0f} ms)")
+        ds.solve_batched(ops[:min(nb, args.bse_workers)], 8, workers=args.bse_workers)  # worker threads warm
+        for order in ("largest_first", "smallest_first"):
+            t0 = time.perf_counter()
+            ev_b, _, infos, sec, status = ds.solve_batched(ops, 8, workers=args.bse_workers, order=order)
+            t_b = time.perf_counter() - t0
+            if not ds.converged():
+                raise RuntimeError(f"batched segment BSE ({p}) did not converge: status {status.tolist()}")
+            diff = max(float(np.abs(ev_b[s] - ev_ser[s]).max()) for s in range(nb))
+            key = f"bse_{tag}_batched" + ("" if order == "largest_first" else "_smallest_first")
+            res.update({f"{key}_s": t_b, f"{key}_segments_per_s": nb / t_b,
+                        f"{key}_completion_ms_p50": pct(sec, 50), f"{key}_completion_ms_p99": pct(sec, 99),
+                        f"{key}_max_abs_diff_vs_serial": diff,
+                        f"{key}_iterations_mean": float(np.mean([i.iterations for i in infos]))})
+            log(f"[segments] BSE-TDA {p} batched ({order}) on {args.bse_workers} workers: {t_b:.2f} s "
+                f"({nb / t_b:.1f} seg/s, completion p50/p99 {pct(sec, 50):.0f}/{pct(sec, 99):.0f} ms); "
+                f"max |dOmega| vs serial {diff:.1e} Ha")
+    for op in ops:
         op.close()
-        t.close()
-        return time.perf_counter() - t0, ev
-
-    t_all = time.perf_counter()
-    serial = [bse_segment(ctx, s) for s in range(nb)]
-    t_serial_bse = time.perf_counter() - t_all
-    lat_bse = [x[0] for x in serial]
-    res.update({"bse_segments": nb, "bse_latency_ms_p50": float(np.percentile(lat_bse, 50) * 1e3),
-                "bse_latency_ms_p99": float(np.percentile(lat_bse, 99) * 1e3),
-                "bse_segments_per_s": nb / t_serial_bse})
-    log(f"[segments] BSE-TDA serial: {nb} segments in {t_serial_bse:.2f} s")
-    if args.bse_streams > 1:
-        import threading
-        from concurrent.futures import ThreadPoolExecutor
-        local = threading.local()
-        ctxs = []
-
-        def work(s):
-            if not hasattr(local, "ctx"):
-                local.ctx = px.Context(0)
-                ctxs.append(local.ctx)
-            return bse_segment(local.ctx, s)
-
-        t_all = time.perf_counter()
-        with ThreadPoolExecutor(max_workers=args.bse_streams) as pool:
-            conc = list(pool.map(work, range(nb)))
-        t_conc = time.perf_counter() - t_all
-        same = max(float(np.abs(a[1] - b[1]).max()) for a, b in zip(serial, conc))
-        lat_c = [x[0] for x in conc]
-        res.update({"bse_streams": args.bse_streams, "bse_concurrent_segments_per_s": nb / t_conc,
-                    "bse_concurrent_latency_ms_p50": float(np.percentile(lat_c, 50) * 1e3),
-                    "bse_concurrent_latency_ms_p99": float(np.percentile(lat_c, 99) * 1e3),
-                    "bse_concurrent_max_abs_diff_vs_serial": same})
-        log(f"[segments] BSE-TDA on {args.bse_streams} streams: {t_conc:.2f} s ({nb / t_conc:.1f} seg/s), "
-            f"max |dOmega| vs serial {same:.1e}")
-        for c in ctxs:
-            c.close()
     print(json.dumps({"workload": f"{nseg} uneven synthetic segments, n_occ U[10,60], n_virt n_occ*U[5,8], n_aux 4(n_occ+n_virt)", "n_segments": nseg, **res}))
 
 
@@ -804,7 +811,8 @@ def main():
                     help="matvec workload: also run a 1 GiB copy kernel (counter calibration)")
     ap.add_argument("--nseg", type=int, default=256)
     ap.add_argument("--bse-segments", type=int, default=32)
-    ap.add_argument("--bse-streams", type=int, default=4, help="segments workload: concurrent BSE solves")
+    ap.add_argument("--bse-workers", type=int, default=4,
+                    help="segments workload: worker streams of the batched BSE solve")
     ap.add_argument("--segments-precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--skip-fp64-solve", action="store_true")
     ap.add_argument("--fp64-reps", type=int, default=3, help="FP64 GPU solves (median reported)")

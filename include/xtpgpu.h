@@ -423,6 +423,22 @@ int xtpgpu_davidson_solve_ham_mf(xtpgpu_mfop R, xtpgpu_mfop C, int64_t neigen,
                                  const xtpgpu_davidson_opts* opts, const double* guess, int64_t ldg,
                                  int64_t nguess, double* evals, double* evecs, int64_t ldv,
                                  xtpgpu_davidson_info* info);
+/* Many segments' BSE-TDA problems, each solved as xtpgpu_davidson_solve would with the same
+ * options: the per-segment jobs that ParallelXJobCalc hands to its worker threads
+ * (parallelxjobcalc.cc:52-111), here `workers` host threads with a HIP stream, a Davidson
+ * workspace and scratch_bytes of operator-product scratch each (0: the largest segment's
+ * single-chunk need, capped at 2 GiB), taking segments in `order`: largest operator first (the
+ * shortest overall time), smallest first (the earliest completions) or as given. evals: nops x
+ * neigen (segment s at evals + s*neigen); evecs (nullable): nops host pointers, each size_s x
+ * neigen column-major (entries may be null); infos, seconds, status (nullable, nops each): the
+ * segment's solve info, the time from the call's start to the segment's completion, and its
+ * return code. Returns the first segment error, else XTPGPU_ERR_NOT_CONVERGED when a segment did
+ * not converge (its roots zeroed as in xtpgpu_davidson_solve), else XTPGPU_OK. */
+enum { XTPGPU_BATCH_LARGEST_FIRST = 0, XTPGPU_BATCH_SMALLEST_FIRST = 1, XTPGPU_BATCH_AS_GIVEN = 2 };
+int xtpgpu_davidson_solve_batched(int64_t nops, const xtpgpu_bseop* ops, int64_t neigen,
+                                  const xtpgpu_davidson_opts* opts, int32_t workers, int32_t order,
+                                  int64_t scratch_bytes, double* evals, double* const* evecs,
+                                  xtpgpu_davidson_info* infos, double* seconds, int32_t* status);
 
 /* ---------------------------------------------------------------- batched segments */
 /* eps(i w) for many segments of uneven size (the per-segment calculate_epsilon_i of
@@ -437,6 +453,8 @@ int xtpgpu_epsbatch_create(xtpgpu_ctx ctx, int nseg, const xtpgpu_mmn* tensors,
                            int grid, xtpgpu_epsbatch* out);
 int xtpgpu_epsbatch_run(xtpgpu_epsbatch b, int use_graph);
 int xtpgpu_epsbatch_set_grid(xtpgpu_epsbatch b, int grid);
+/* Time the batch at 1, 2, 4 and 8 workgroups per CU and keep the fastest grid (grid_out, nullable). */
+int xtpgpu_epsbatch_autotune(xtpgpu_epsbatch b, int* grid_out);
 int xtpgpu_epsbatch_result(xtpgpu_epsbatch b, int seg, double* out);
 int xtpgpu_epsbatch_destroy(xtpgpu_epsbatch b);
 

@@ -296,6 +296,22 @@ class DavidsonSolver {
     return finish(xtpgpu_davidson_solve_ham_mf(R.mfop(), C.mfop(), neigen, &o_, nullptr, 0, 0, eval_.data(),
                                                evec_.data(), n_, &info_));
   }
+  // many segments' TDA problems (the jobs ParallelXJobCalc runs one per thread,
+  // parallelxjobcalc.cc:52-111) on `workers` streams in `order`: ops are BSEOperator::get()
+  // handles; evals gets one row of neigen per operator, seconds (optional) each segment's completion
+  // time from the call's start. Returns false when a segment did not converge.
+  bool solve_batched(const std::vector<xtpgpu_bseop>& ops, int64_t neigen, int workers,
+                     std::vector<std::vector<double>>& evals, std::vector<double>* seconds = nullptr,
+                     int order = XTPGPU_BATCH_LARGEST_FIRST) {
+    const size_t n = ops.size();
+    std::vector<double> ev(n * neigen, 0.0), sec(n, 0.0);
+    const int rc = xtpgpu_davidson_solve_batched((int64_t)n, ops.data(), neigen, &o_, workers, order, 0, ev.data(),
+                                                 nullptr, nullptr, sec.data(), nullptr);
+    evals.assign(n, {});
+    for (size_t s = 0; s < n; ++s) evals[s].assign(ev.begin() + s * neigen, ev.begin() + (s + 1) * neigen);
+    if (seconds) *seconds = sec;
+    return finish(rc);
+  }
   // full BSE through the HamiltonianOperator [[A, B], [-B, -A]] (BSE::Solve_nonhermitian_Davidson)
   template <class OpA, class OpB>
   bool solve_full(const OpA& A, const OpB& B, int64_t neigen) {

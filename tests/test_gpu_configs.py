@@ -256,6 +256,7 @@ def test_segments_eps_batch_reduced_law(ctx, prec, tol):
         M = t.get_tensor()
         refs.append([X.rpa_epsilon(M, e, o - 1, 0, o + v - 1, f, "imag") for f in (freqs[0], freqs[7])])
     batch = px.EpsBatch(rpas, freqs, prec)
+    assert batch.autotune() in (256 * m for m in (1, 2, 4, 8))
     for _ in range(2):
         batch.run(True)
         for s in range(len(segs)):
@@ -293,12 +294,59 @@ def test_segments_tda_reduced_law(ctx):
         t.close()
 
 
+def test_segments_tda_batched(ctx):
+    """The same law's 16 segments solved together (xtpgpu_davidson_solve_batched: worker threads
+    with their own streams, largest segment first) in FP64 and in the mixed BF16x3 form: every
+    segment's 8 roots against dense eigh (1e-7 Ha) and against its own single-segment solve (1e-10
+    Ha), with one and four workers dealt largest first and three dealt smallest first; eigenvectors
+    against the single solve's up to sign; a scratch region below one product's need is refused
+    before any solve."""
+    segs = _segments(33, 16)
+    ops, ts, dense = [], [], []
+    for s, (o, v, P) in enumerate(segs):
+        n = o + v
+        t = px.TCMatrix_gwbse(ctx).Initialize(P, 0, n - 1, 0, n - 1)
+        t.fill_synthetic(300 + s, 0.01, 3.0)
+        e = X.synthetic_energies(300 + s, o, v)
+        epsinv = px.setup_screening(t, e, o - 1, 0, n - 1)
+        dense.append(np.linalg.eigvalsh(X.bse_dense(1, 2, 1, 0, epsinv, t.get_tensor(), np.diag(e), o - 1, 0, 0,
+                                                    n - 1))[:8])
+        ops.append(px.SingletOperator_TDA(epsinv, t, np.diag(e), precisions=("fp64", "bf16x3")).configure(
+            px.BSEOperator_Options(o - 1, 0, 0, 0, n - 1)))
+        ts.append(t)
+    for prec in ("fp64", "bf16x3"):
+        ds = px.DavidsonSolver()
+        ds.set_tolerance(1e-8)
+        ds.set_iter_max(200)
+        ds.set_max_search_space(80)
+        if prec != "fp64":
+            ds.set_precision(prec, refine_tolerance=1e-8, refine_max=8)
+        single = [ds.solve(op, 8, want_vectors=True) for op in ops]
+        for workers, order in ((1, "largest_first"), (4, "largest_first"), (3, "smallest_first")):
+            ev, vecs, infos, sec, status = ds.solve_batched(ops, 8, workers=workers, want_vectors=True, order=order)
+            assert ds.converged() and (status == 0).all(), (prec, workers, status)
+            assert (sec > 0).all() and len(infos) == len(ops)
+            for s in range(len(ops)):
+                assert np.abs(ev[s] - dense[s]).max() < 1e-7, (prec, workers, s, np.abs(ev[s] - dense[s]).max())
+                assert np.abs(ev[s] - single[s][0]).max() < 1e-10, (prec, workers, s)
+                ov = np.abs(np.sum(vecs[s] * single[s][1], axis=0))
+                assert np.abs(ov - 1.0).max() < 1e-6, (prec, workers, s, ov)
+                assert infos[s].converged == 1
+    with pytest.raises(px.XtpGpuError, match="scratch_bytes"):
+        px.DavidsonSolver().solve_batched(ops, 8, workers=2, scratch_bytes=4096)
+    for op, t in zip(ops, ts):
+        op.close()
+        t.close()
+
+
 # ------------------------------------------------------------------ large segment (reduced)
 def test_large_segment_reduced_law(ctx):
     """The large-segment workload's law at n_occ=10, n_virt=90, n_aux=240: Sigma_c^CDA with 20
     Legendre points on a 20-level QP window around the gap against the oracle's Sigma_CDA
     (sigma0=0.3; 1e-9 Ha), then a 50-root singlet TDA (sigma0=0.01, search space 500) in FP64 and
-    in the mixed form against dense eigh (1e-7 Ha)."""
+    in the mixed form against dense eigh (1e-7 Ha), and in the form the full-size workload runs
+    (operator packed from the synthetic source, FP64 M_cc' streamed, full eps^-1(0) screening,
+    BF16 -> BF16x3 ladder with FP64 refinement)."""
     o, v, P = 10, 90, 240
     n = o + v
     t = px.TCMatrix_gwbse(ctx).Initialize(P, 0, n - 1, 0, n - 1)
@@ -339,3 +387,24 @@ def test_large_segment_reduced_law(ctx):
         assert np.abs(ev - w).max() < 1e-7, (prec, np.abs(ev - w).max())
     op.close()
     t.close()
+    # the large workload's form: packed from the synthetic source in aux chunks, FP64 M_cc' streamed,
+    # unrotated slices screened by the full eps^-1(0); the same 50 roots
+    t2 = px.TCMatrix_gwbse(ctx).Initialize(P, 0, n - 1, 0, n - 1)
+    t2.fill_synthetic(42, 0.01, 3.0)
+    rpa2 = px.RPA(t2)
+    rpa2.configure(o - 1, 0, n - 1)
+    rpa2.setRPAInputEnergies(e)
+    lam, U = np.linalg.eigh(rpa2.calculate_epsilon_r(0.0))
+    W = (U / lam) @ U.T
+    t2.close()
+    stm = px.SingletOperator_TDA(None, None, np.diag(e), precisions=("fp64", "bf16", "bf16x3")).configure_streamed(
+        px.BSEOperator_Options(o - 1, 0, 0, 0, n - 1), px.SyntheticSource(42, 0.01, 3.0, P), ctx,
+        stream_fp64=True, chunk_bytes=50 * n * n * 8, epsinv_matrix=0.5 * (W + W.T))
+    ds = px.DavidsonSolver()
+    ds.set_tolerance(1e-7)
+    ds.set_iter_max(300)
+    ds.set_max_search_space(500)  # the workload's 20 x roots would span all N = 900 dimensions here
+    ds.set_precision("bf16x3", refine_tolerance=1e-7, refine_max=8, start="bf16", switch_tolerance=1e-5)
+    ev, _ = ds.solve(stm, 50, want_vectors=False)
+    assert ds.converged()
+    assert np.abs(ev - w).max() < 1e-7, np.abs(ev - w).max()

@@ -164,13 +164,25 @@ inline bool sym_eig_lowest(int n, const double* A, int lda, int need, double* w,
     for (int j = k; j < n; ++j) {
       const double* col = &a[(size_t)j * n];
       const double vj = v[j];
-      double dot = 0.0;
+      double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;  // four dot-product chains
       p[j] += col[j] * vj;
-      for (int i = j + 1; i < n; ++i) {
-        p[i] += col[i] * vj;
-        dot += col[i] * v[i];
+      int i = j + 1;
+      for (; i + 3 < n; i += 4) {
+        const double c0 = col[i], c1 = col[i + 1], c2 = col[i + 2], c3 = col[i + 3];
+        p[i] += c0 * vj;
+        p[i + 1] += c1 * vj;
+        p[i + 2] += c2 * vj;
+        p[i + 3] += c3 * vj;
+        d0 += c0 * v[i];
+        d1 += c1 * v[i + 1];
+        d2 += c2 * v[i + 2];
+        d3 += c3 * v[i + 3];
       }
-      p[j] += dot;
+      for (; i < n; ++i) {
+        p[i] += col[i] * vj;
+        d0 += col[i] * v[i];
+      }
+      p[j] += (d0 + d1) + (d2 + d3);
     }
     for (int i = k; i < n; ++i) p[i] *= beta;
     double pv = 0.0;
@@ -212,7 +224,7 @@ inline bool sym_eig_lowest(int n, const double* A, int lda, int need, double* w,
       double mu = dd[h] - (den != 0.0 ? e2 / den : 0.0);
       double x = dd[l] - mu, z = ee[l];
       for (int k = l; k < h; ++k) {
-        double r = std::hypot(x, z);
+        double r = std::sqrt(x * x + z * z);  // entries are O(|T|): no overflow to guard against
         double cc = 1.0, ss = 0.0;
         if (r != 0.0) { cc = x / r; ss = z / r; }
         if (k > l) ee[k - 1] = r;
@@ -317,17 +329,38 @@ inline bool sym_eig_lowest(int n, const double* A, int lda, int need, double* w,
     w[j] = dd[j];
   }
   // back-transform: Z = H_0 H_1 ... H_{n-3} Y
-  for (int j = 0; j < need; ++j) {
-    double* z = &Y[(size_t)j * n];
+  // four vectors at a time: each reflector is read once for four independent dot-product chains
+  for (int j0 = 0; j0 < need; j0 += 4) {
+    const int nj = std::min(4, need - j0);
+    double* z[4];
+    for (int t = 0; t < 4; ++t) z[t] = &Y[(size_t)(j0 + std::min(t, nj - 1)) * n];  // spare lanes alias the last
     for (int k = n - 3; k >= 0; --k) {
       if (betas[k] == 0.0) continue;
       const double* v = &refl[(size_t)k * n];
-      double s = 0.0;
-      for (int i = k + 1; i < n; ++i) s += v[i] * z[i];
-      s *= betas[k];
-      for (int i = k + 1; i < n; ++i) z[i] -= s * v[i];
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      for (int i = k + 1; i < n; ++i) {
+        const double vi = v[i];
+        s0 += vi * z[0][i];
+        s1 += vi * z[1][i];
+        s2 += vi * z[2][i];
+        s3 += vi * z[3][i];
+      }
+      const double ss[4] = {s0 * betas[k], s1 * betas[k], s2 * betas[k], s3 * betas[k]};
+      if (nj == 4) {
+        for (int i = k + 1; i < n; ++i) {
+          const double vi = v[i];
+          z[0][i] -= ss[0] * vi;
+          z[1][i] -= ss[1] * vi;
+          z[2][i] -= ss[2] * vi;
+          z[3][i] -= ss[3] * vi;
+        }
+      } else {
+        for (int t = 0; t < nj; ++t)
+          for (int i = k + 1; i < n; ++i) z[t][i] -= ss[t] * v[i];
+      }
     }
-    for (int i = 0; i < n; ++i) Z[(size_t)j * n + i] = z[i];
+    for (int t = 0; t < nj; ++t)
+      for (int i = 0; i < n; ++i) Z[(size_t)(j0 + t) * n + i] = z[t][i];
   }
   return true;
 }

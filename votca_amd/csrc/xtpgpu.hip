@@ -11,6 +11,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -23,6 +24,7 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "gemm_kernels.h"
@@ -860,7 +862,8 @@ void rpa_epsilon_impl(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* energies, in
     const long gmax = std::max<long>(1, std::min<long>(n_omega, (long)(eps_scale_budget() / per_w)));
     const long ngroups = (n_omega + gmax - 1) / gmax;
     const long gsz = (n_omega + ngroups - 1) / ngroups;
-    DevBuf sc(per_w * gsz);
+    DevBuf& sc = ctx->ws[1];  // kept on the context: a multi-GB allocation and free per build costs more than the build
+    sc.ensure(per_w * gsz);
     double* Bs = sc.as<double>();
     const double* A64 = pk.A64.as<double>();
     const Opnd<double> A{A64, nullptr, Kp, 0};
@@ -887,7 +890,8 @@ void rpa_epsilon_impl(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* energies, in
     const long gmax = std::max<long>(1, std::min<long>(n_omega, (long)(eps_scale_budget() / per_w)));
     const long ngroups = (n_omega + gmax - 1) / gmax;
     const long gsz = (n_omega + ngroups - 1) / ngroups;
-    DevBuf sc(per_w * gsz);
+    DevBuf& sc = ctx->ws[1];
+    sc.ensure(per_w * gsz);
     unsigned short* Bh = sc.as<unsigned short>();
     unsigned short* Bl = split ? Bh + plane * gsz : nullptr;
     const unsigned short* Ah = pk.Ah.as<unsigned short>();
@@ -1069,10 +1073,12 @@ MatmulPlan plan_matmul(const xtpgpu_bseop_s* op, int precision, long k) {
 // operator's grow-only buffers, or, with `arena`, from that fixed region only (no allocation,
 // no profiling events; the stage-1 aux chunk shrinks to what the region has left).
 void bse_matmul_impl(xtpgpu_bseop op, int precision, const double* X, long ldx, long k, double* Y, long ldy,
-                     size_t budget = hd_chunk_budget(), ScratchArena* arena = nullptr) {
+                     size_t budget = hd_chunk_budget(), ScratchArena* arena = nullptr, hipStream_t on = nullptr) {
   xtpgpu_ctx ctx = op->ctx;
-  xtpgpu_ctx pctx = arena ? nullptr : ctx;  // ProfScope is inert without a context
-  hipStream_t st = ctx->stream;
+  // ProfScope records on the context's stream: inert without a context, so off for an arena call
+  // and for work issued on another stream
+  xtpgpu_ctx pctx = (arena || on) ? nullptr : ctx;
+  hipStream_t st = on ? on : ctx->stream;
   const long N = op->N, vt = op->vt, ct = op->ct, P = op->P;
   const long ctp = op->ctp, vtp = op->vtp, Pp = op->Pp;
   const long Np = vt * ctp;  // padded length of one input vector

@@ -85,7 +85,8 @@ struct KTimer {
 struct xtpgpu_ctx_s {
   int device = 0;
   hipStream_t stream = nullptr;
-  // grow-only scratch buffers, reused across calls on this context
+  // grow-only scratch buffers, reused across calls on this context (ws[1]: the eps builds' scaled
+  // transition copies, at most the scale budget)
   xg::DevBuf ws[6];
   bool profiling = false;
   void* blas = nullptr;  // rocblas_handle for the rocSOLVER calls, bound to `stream`

@@ -228,6 +228,9 @@ SIGNATURES = {
                                            C.POINTER(DavidsonInfo)]),
     "xtpgpu_davidson_solve_ham_mf": (C.c_int, [_vp, _vp, _i64, C.POINTER(DavidsonOpts), _dp, _i64, _i64, _dp,
                                                _dp, _i64, C.POINTER(DavidsonInfo)]),
+    "xtpgpu_davidson_solve_batched": (C.c_int, [_i64, C.POINTER(_vp), _i64, C.POINTER(DavidsonOpts), C.c_int32,
+                                                C.c_int32, _i64, _dp, C.POINTER(_dp), C.POINTER(DavidsonInfo), _dp,
+                                                C.POINTER(C.c_int32)]),
     "xtpgpu_bse_adjoint": (C.c_int, [_vp, _vp, _i64, _dp, _dp, _i64, _dp, _dp]),
     "xtpgpu_bse_adjoint_device": (C.c_int, [_vp, _vp, _i64, _dp, _dp, _i64, _dp, _vp]),
     "xtpgpu_bse_adjoint_weighted_device": (C.c_int, [_vp, _vp, _i64, _dp, _dp, _i64, _dp, _dp, _vp]),
@@ -242,6 +245,7 @@ SIGNATURES = {
                                          C.POINTER(_vp)]),
     "xtpgpu_epsbatch_run": (C.c_int, [_vp, C.c_int]),
     "xtpgpu_epsbatch_set_grid": (C.c_int, [_vp, C.c_int]),
+    "xtpgpu_epsbatch_autotune": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "xtpgpu_epsbatch_result": (C.c_int, [_vp, C.c_int, _dp]),
     "xtpgpu_epsbatch_destroy": (C.c_int, [_vp]),
     "xtpgpu_bse_setup_screening": (C.c_int, [_vp, _vp, _dp, _i64, _i64, _i64, C.c_double, _dp]),
@@ -947,6 +951,35 @@ class DavidsonSolver:
                                          _dptr(vec) if want_vectors else None, op.size, C.byref(self.info))
         return self._finish(rc, ev, vec)
 
+    _ORDER = {"largest_first": 0, "smallest_first": 1, "as_given": 2}
+
+    def solve_batched(self, ops, neigen: int, workers: int = 4, want_vectors=False, scratch_bytes: int = 0,
+                      order: str = "largest_first"):
+        """Many segments' TDA problems (the per-segment jobs of ParallelXJobCalc,
+        parallelxjobcalc.cc:52-111) with this solver's options, on `workers` streams, segments dealt
+        largest operator first, smallest first or as given. Returns (evals (nops x neigen),
+        vectors (list, or None), infos, seconds: each segment's completion time from the call's
+        start, status: per-segment return codes)."""
+        if order not in self._ORDER:
+            raise ValueError(order + " is not a batch order (largest_first, smallest_first, as_given)")
+        ops = list(ops)
+        n = len(ops)
+        if n == 0:
+            raise ValueError("solve_batched needs at least one operator")
+        hs = (_vp * n)(*[o.h for o in ops])
+        ev = np.zeros((n, neigen))
+        vecs = [np.zeros((o.size, neigen), order="F") for o in ops] if want_vectors else None
+        vp = (_dp * n)(*[_dptr(v) for v in vecs]) if want_vectors else None
+        infos = (DavidsonInfo * n)()
+        sec = np.zeros(n)
+        status = (C.c_int32 * n)()
+        rc = lib().xtpgpu_davidson_solve_batched(n, hs, neigen, C.byref(self.opts), int(workers), self._ORDER[order],
+                                                 int(scratch_bytes), _dptr(ev), vp, infos, _dptr(sec), status)
+        self._converged = rc == 0
+        if rc not in (0, -4):
+            _check(rc)
+        return ev, vecs, list(infos), sec, np.array(status[:], dtype=np.int32)
+
     def _finish(self, rc, ev, vec):
         self._eval, self._evec = ev, vec
         self._converged = rc == 0
@@ -1414,6 +1447,12 @@ class EpsBatch:
 
     def set_grid(self, grid):
         _check(lib().xtpgpu_epsbatch_set_grid(self.h, int(grid)))
+
+    def autotune(self) -> int:
+        """Keep the fastest persistent grid of 1, 2, 4, 8 workgroups per CU (timed in the library)."""
+        g = C.c_int(0)
+        _check(lib().xtpgpu_epsbatch_autotune(self.h, C.byref(g)))
+        return g.value
 
     def result(self, seg):
         P = self.rpas[seg].Mmn.aux
