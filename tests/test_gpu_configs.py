@@ -237,11 +237,16 @@ def _segments(seed, nseg):
     return segs
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("bf16x3", 2e-5)])
-def test_segments_eps_batch_reduced_law(ctx, prec, tol):
+@pytest.mark.parametrize("form", ["scaled", "load"])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("bf16x3", 2e-5), ("bf16", 1e-2)])
+def test_segments_eps_batch_reduced_law(ctx, prec, tol, form, monkeypatch):
     """16 uneven segments on the batched-segment law (n_occ U[6,12], n_virt = n_occ U[5,8],
     n_aux = 4(n_occ+n_virt), sigma0=0.3): eps(i w) at the 12 modified Gauss-Legendre frequencies for
-    all segments in one persistent launch, hipGraph-replayed twice, against the oracle per segment."""
+    all segments in one persistent launch, hipGraph-replayed twice, against the oracle per segment;
+    in the scaled form (per-frequency passes over scaled transition copies, here forced to several
+    passes per frequency by a small scratch budget) and with the weights applied on the operand load."""
+    monkeypatch.setenv("XTPGPU_EPSBATCH_SCALED", "1" if form == "scaled" else "0")
+    monkeypatch.setenv("XTPGPU_EPS_SCALE_BYTES", str(3 << 20))
     segs = _segments(31, 16)
     freqs = quadrature.create("modified_legendre", 12).points()
     rpas, refs = [], []
