@@ -846,10 +846,10 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="matvec workload: also run a 1 GiB copy kernel (counter calibration)")
     ap.add_argument("--nseg", type=int, default=256)
-    ap.add_argument("--bse-segments", type=int, default=32, help="segments workload: segments solved by the BSE")
+    ap.add_argument("--bse-segments", type=int, default=256, help="segments workload: segments solved by the BSE")
     ap.add_argument("--bse-wave", type=int, default=32,
                     help="segments workload: BSE operators resident at once (segments are set up and solved in waves)")
-    ap.add_argument("--bse-workers", type=int, default=4,
+    ap.add_argument("--bse-workers", type=int, default=8,
                     help="segments workload: worker streams of the batched BSE solve")
     ap.add_argument("--segments-precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--skip-fp64-solve", action="store_true")
