@@ -606,6 +606,7 @@ def run_large_cda(px, ctx, args, e, o, v, P):
     rpa.setRPAInputEnergies(e)
     cda = px.Sigma_CDA(t, rpa).configure(o - 1, qpmin, qpmax, 0, n - 1, eta=1e-3, alpha=1e-3,
                                          quadrature_scheme="legendre", order=20, screening_precision=args.mixed)
+    log(f"[large] Sigma_c^CDA on levels {qpmin}-{qpmax}: preparing the imaginary-axis screening")
     t0 = time.perf_counter()
     cda.PrepareScreening()
     prep = time.perf_counter() - t0
@@ -625,9 +626,12 @@ def run_large_cda(px, ctx, args, e, o, v, P):
 
     nres = residues(qpmin, qpmax)
     n100 = residues(o - 50, o + 49)
+    log(f"[large] Sigma_c^CDA prepared in {prep:.1f} s, {2 * L} intercepts evaluated in {ev:.1f} s; QP solve:")
+    os.environ["XTPGPU_QP_LOG"] = "1"  # one line per served batch of Sigma requests on stderr
     t0 = time.perf_counter()
     qp, conv = cda.SolveQP(icpt, icpt.copy(), qp_solver="fixedpoint")
     t_qp = time.perf_counter() - t0
+    os.environ["XTPGPU_QP_LOG"] = "0"
     out = {"levels": [qpmin, qpmax], "sigma0": args.large_gw_sigma0, "prepare_s": prep, "eval_s": ev,
            "residue_builds": nres, "s_per_residue_build": ev / max(nres, 1),
            "residue_builds_100_level_window": n100, "sigma_c_at_intercepts_Ha": [float(x) for x in sig],

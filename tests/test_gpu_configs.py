@@ -319,6 +319,9 @@ def test_segments_tda_batched(ctx):
         ops.append(px.SingletOperator_TDA(epsinv, t, np.diag(e), precisions=("fp64", "bf16x3")).configure(
             px.BSEOperator_Options(o - 1, 0, 0, 0, n - 1)))
         ts.append(t)
+        if s == 0:  # the same segment's operator without the mixed forms
+            fp64_only = px.SingletOperator_TDA(epsinv, t, np.diag(e)).configure(
+                px.BSEOperator_Options(o - 1, 0, 0, 0, n - 1))
     for prec in ("fp64", "bf16x3"):
         ds = px.DavidsonSolver()
         ds.set_tolerance(1e-8)
@@ -339,6 +342,17 @@ def test_segments_tda_batched(ctx):
                 assert infos[s].converged == 1
     with pytest.raises(px.XtpGpuError, match="scratch_bytes"):
         px.DavidsonSolver().solve_batched(ops, 8, workers=2, scratch_bytes=4096)
+    # per-segment outcomes: an iteration cap leaves every segment unconverged (a status, not an
+    # error); an operator without the asked precision fails its segment, which the error names
+    ds = px.DavidsonSolver()
+    ds.set_iter_max(2)
+    ev, _, infos, sec, status = ds.solve_batched(ops[:5], 8, workers=3)
+    assert not ds.converged() and (status == -4).all() and all(i.converged == 0 for i in infos)
+    ds = px.DavidsonSolver()
+    ds.set_precision("bf16x3", refine_tolerance=1e-6)
+    with pytest.raises(px.XtpGpuError, match="segment 2: .*without this precision"):
+        ds.solve_batched([ops[1], ops[2], fp64_only], 8, workers=2)
+    fp64_only.close()
     for op, t in zip(ops, ts):
         op.close()
         t.close()

@@ -16,9 +16,13 @@
 #pragma once
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <condition_variable>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <exception>
 #include <functional>
 #include <limits>
@@ -85,8 +89,12 @@ class Batcher {
     double* s;
     double* ds;
   };
-  // called with the lock held and every live worker waiting
+  // called with the lock held and every live worker waiting; with XTPGPU_QP_LOG set, one line per
+  // served batch on stderr (a long QP solve then shows its progress)
   void run() {
+    const char* lg = std::getenv("XTPGPU_QP_LOG");
+    const bool log = lg && *lg && std::strcmp(lg, "0") != 0;
+    const auto t0 = std::chrono::steady_clock::now();
     try {
       for (int pass = 0; pass < 2; ++pass) {  // value-only requests, then those with a derivative
         std::vector<long> lv;
@@ -108,6 +116,11 @@ class Batcher {
       }
     } catch (...) {
       err_ = std::current_exception();
+    }
+    if (log) {
+      const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      std::fprintf(stderr, "[qp] batch %llu: %zu Sigma requests, %.2f s\n", (unsigned long long)gen_, pend_.size(), dt);
+      std::fflush(stderr);
     }
     pend_.clear();
     ++gen_;
