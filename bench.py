@@ -750,14 +750,33 @@ def run_adjoint(args):
     steps = (1.0, -1.0) if args.fd_order == 2 else (1.0, -1.0, 2.0, -2.0)
     checks = []
     t_fd = time.perf_counter()
+
+    def solve_at(sgn, near):
+        """The perturbed problem at step sgn, started from the converged vectors of the nearest
+        evaluated step of the same sign (continuation) topped up with the base solve's higher roots,
+        or from the base solve's vectors; if the harmonic Ritz step fails from one start (the
+        small nonsymmetric eigenproblem can stall, as the reference's does), from the other."""
+        starts = [warm]
+        if abs(sgn) > 1.0 and sgn / 2.0 in near:
+            xn, yn = near[sgn / 2.0]
+            starts.insert(0, np.hstack([np.vstack([xn, yn]), warm[:, na:]]))
+        for k, g in enumerate(starts):
+            try:
+                return solve(g, args.fd_tol, neigen=na)
+            except (px.XtpGpuError, RuntimeError) as exc:
+                if k == len(starts) - 1:
+                    raise
+                log(f"[adjoint] step {sgn:+.0f}h: {exc}; restarting from the base vectors")
+
     for kind, where in todo:
-        w = {}
+        w, near = {}, {}
         if kind == "e_QP":
             for sgn in steps:
                 Hp = H0.copy()
                 Hp[where, where] += sgn * h
                 A.set_hqp(Hp)
-                _, xp_, yp_, wt, inf = solve(warm, args.fd_tol, neigen=na)
+                _, xp_, yp_, wt, inf = solve_at(sgn, near)
+                near[sgn] = (xp_, yp_)
                 w[sgn] = stationary(xp_, yp_)
             A.set_hqp(H0)
             adj = d_eps[where, :na].copy()
@@ -770,7 +789,8 @@ def run_adjoint(args):
                 Mmn.set_block(v_, o + c_, m.reshape(P, 1, 1), symmetric=True)
                 A.refresh_transitions()
                 B.refresh_transitions()
-                _, xp_, yp_, wt, inf = solve(warm, args.fd_tol, neigen=na)
+                _, xp_, yp_, wt, inf = solve_at(sgn, near)
+                near[sgn] = (xp_, yp_)
                 w[sgn] = stationary(xp_, yp_)
             Mmn.set_block(v_, o + c_, col.reshape(P, 1, 1), symmetric=True)
             A.refresh_transitions()

@@ -218,6 +218,12 @@ int xtpgpu_bse_matmul_device_noalloc(xtpgpu_bseop op, int precision, const doubl
 /* ---------------------------------------------------------------- screening, GW */
 typedef struct xtpgpu_sigma_s* xtpgpu_sigma;
 
+/* AOCoulomb::Pseudo_InvSqrt_GWBSE (aomatrix.cc:53-86): V (aux Coulomb) and S (aux overlap), n x n
+ * column-major host matrices -> out = ((S^-1/2 V S^-1/2)^-1/2 S^-1/2)^T, the right factor
+ * TCMatrix_gwbse::Fill applies (threecenter.cc:72-90); eigenvalues below etol (5e-7 in the
+ * reference) of either eigendecomposition are dropped and counted in removed (nullable). */
+int xtpgpu_aux_pseudo_invsqrt(xtpgpu_ctx ctx, const double* V, const double* S, int64_t n, double etol,
+                              double* out, int64_t* removed);
 /* BSE::SetupDirectInteractionOperator (bse.cc:188-204): eps_r(energy) on the device, its
  * eigendecomposition (rocSOLVER), Mmn <- Mmn * U in place, eps0inv[P] = 1/lambda_P (0 below 1e-8). */
 int xtpgpu_bse_setup_screening(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* energies, int64_t homo,

@@ -130,6 +130,18 @@ class HostMatrixFreeOperator : public MatrixFreeOperator {
   }
 };
 
+// AOCoulomb::Pseudo_InvSqrt_GWBSE (aomatrix.cc:53-86): V and S are n x n column-major (aux Coulomb,
+// aux overlap); returns ((S^-1/2 V S^-1/2)^-1/2 S^-1/2)^T, the factor TCMatrix::MultiplyRightWithAuxMatrix
+// applies in Fill; removedfunctions counts the eigenvalues below etol.
+inline std::vector<double> Pseudo_InvSqrt_GWBSE(Context& ctx, const std::vector<double>& V, const std::vector<double>& S,
+                                                int64_t n, double etol = 5e-7, int64_t* removedfunctions = nullptr) {
+  if (n <= 0 || V.size() != (size_t)(n * n) || S.size() != (size_t)(n * n))
+    throw std::runtime_error("Pseudo_InvSqrt_GWBSE: V and S must both hold n x n values");
+  std::vector<double> out((size_t)(n * n));
+  check(xtpgpu_aux_pseudo_invsqrt(ctx.get(), V.data(), S.data(), n, etol, out.data(), removedfunctions));
+  return out;
+}
+
 struct BSEOperatorOptions {  // BSEOperator_Options (bse_operator.h:32-38)
   int64_t homo, rpamin, qpmin, vmin, cmax;
 };

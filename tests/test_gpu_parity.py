@@ -63,11 +63,37 @@ def test_multiply_right(ctx):
     assert relerr(t.get_tensor(), X.multiply_right(M, A)) < 1e-13
 
 
+def test_pseudo_invsqrt_gwbse_in_the_library(ctx, ints):  # aomatrix.cc:53-86, test_threecenter_gwbse.cc:36-126
+    """AOCoulomb::Pseudo_InvSqrt_GWBSE on the device: with no function removed its result R obeys
+    R^T V R = 1 and R R^T = V^-1 (closed forms of ((S^-1/2 V S^-1/2)^-1/2 S^-1/2)^T); it equals the
+    oracle's restatement; TCMatrix_gwbse::Fill through it reproduces the reference's fixtures; an
+    overlap with an eigenvalue below etol has that function removed and counted."""
+    S, Saux, V, T = ints
+    R, removed = px.Pseudo_InvSqrt_GWBSE(V, Saux, 5e-7, ctx)
+    assert removed == 0
+    n = V.shape[0]
+    assert np.abs(R.T @ V @ R - np.eye(n)).max() < 1e-10
+    assert relerr(R @ R.T, np.linalg.inv(V)) < 1e-10
+    assert relerr(R, X.pseudo_invsqrt_gwbse(V, Saux, 5e-7)) < 1e-12
+    C = mmio.read_matrix(os.path.join(G, "threecenter_gwbse/MOs.mm"))
+    t = px.TCMatrix_gwbse(ctx).Initialize(17, 0, 5, 0, 7)
+    t.Fill(T, C, R)
+    M = t.get_tensor()
+    for k in (0, 2, 4):
+        assert mmio.isapprox(M[k], mmio.read_matrix(os.path.join(G, f"threecenter_gwbse/ref{k}b.mm")), 1e-5)
+    lam, U = np.linalg.eigh(Saux)
+    lam[0] = 1e-9  # one aux combination (nearly) linearly dependent
+    S2 = (U * lam) @ U.T
+    R2, removed2 = px.Pseudo_InvSqrt_GWBSE(V, S2, 5e-7, ctx)
+    assert removed2 >= 1
+    assert relerr(R2, X.pseudo_invsqrt_gwbse(V, S2, 5e-7)) < 1e-10
+
+
 def test_fill_ao2mo_matches_threecenter_fixture(ctx, ints):  # test_threecenter_gwbse.cc:36-126
     S, Saux, V, T = ints
     C = mmio.read_matrix(os.path.join(G, "threecenter_gwbse/MOs.mm"))
     t = px.TCMatrix_gwbse(ctx).Initialize(17, 0, 5, 0, 7)
-    t.Fill(T, C, X.pseudo_invsqrt_gwbse(V, Saux))
+    t.Fill(T, C, px.Pseudo_InvSqrt_GWBSE(V, Saux, 5e-7, ctx)[0])
     M = t.get_tensor()
     assert relerr(M, X.fill_mmn(T, C, 0, 5, 0, 7, Saux, V)) < 1e-12
     for k in (0, 2, 4):
@@ -183,7 +209,7 @@ def test_bse_operator_fixtures_gpu(ctx, ints, name, c):  # test_bse_operator.cc:
     C = mmio.read_matrix(os.path.join(G, "bse_operator/MOs.mm"))
     Hqp = mmio.read_matrix(os.path.join(G, "bse_operator/Hqp.mm"))
     t = px.TCMatrix_gwbse(ctx).Initialize(17, 0, 16, 0, 16)
-    t.Fill(T, C, X.pseudo_invsqrt_gwbse(V, Saux))
+    t.Fill(T, C, px.Pseudo_InvSqrt_GWBSE(V, Saux, 5e-7, ctx)[0])
     t.MultiplyRightWithAuxMatrix(mmio.read_matrix(os.path.join(G, "bse_operator/rpa_op.mm")))
     op = px.BSE_OPERATOR(*c, EPS_INV_BSE_OP, t, Hqp).configure(px.BSEOperator_Options(4, 0, 0, 0, 8))
     H = op.matmul(np.eye(op.size))
@@ -211,7 +237,7 @@ def test_bse_energy_fixtures_gpu(ctx, ints, name, psi, c, offdiag, nroots, full)
     C = mmio.read_matrix(os.path.join(G, "bse/MOs.mm"))
     Hqp = mmio.read_matrix(os.path.join(G, "bse/Hqp.mm"))
     t = px.TCMatrix_gwbse(ctx).Initialize(17, 0, 16, 0, 16)
-    t.Fill(T, C, X.pseudo_invsqrt_gwbse(V, Saux))
+    t.Fill(T, C, px.Pseudo_InvSqrt_GWBSE(V, Saux, 5e-7, ctx)[0])
     epsinv = px.setup_screening(t, np.diag(Hqp).copy(), 4, 0, 16)
     Hq = Hqp if offdiag else np.diag(np.diag(Hqp))
     opt = px.BSEOperator_Options(4, 0, 0, 0, 16)
@@ -249,7 +275,7 @@ def test_bse_adjust_hqp_fixtures_gpu(ctx, ints):  # test_bse.cc:342-374
     Hqp = mmio.read_matrix(os.path.join(G, "bse/Hqp.mm"))
     rpa_e = np.diag(Hqp).copy()
     t = px.TCMatrix_gwbse(ctx).Initialize(17, 0, 16, 0, 16)
-    t.Fill(T, C, X.pseudo_invsqrt_gwbse(V, Saux))
+    t.Fill(T, C, px.Pseudo_InvSqrt_GWBSE(V, Saux, 5e-7, ctx)[0])
     bse = px.BSE(t).configure(px.BSE_options(4, 0, 16, 0, 16, 1, 15), rpa_e, Hqp)
     ref_cut = mmio.read_matrix(os.path.join(G, "bse/Hqp_cut.mm"))
     assert mmio.isapprox(bse.getHqp(), ref_cut, 1e-3)
@@ -276,7 +302,7 @@ def test_full_bse_initial_guess_gpu(ctx, ints):
     C = mmio.read_matrix(os.path.join(G, "bse/MOs.mm"))
     Hqp = mmio.read_matrix(os.path.join(G, "bse/Hqp.mm"))
     t = px.TCMatrix_gwbse(ctx).Initialize(17, 0, 16, 0, 16)
-    t.Fill(T, C, X.pseudo_invsqrt_gwbse(V, Saux))
+    t.Fill(T, C, px.Pseudo_InvSqrt_GWBSE(V, Saux, 5e-7, ctx)[0])
     epsinv = px.setup_screening(t, np.diag(Hqp).copy(), 4, 0, 16)
     opt = px.BSEOperator_Options(4, 0, 0, 0, 16)
     A = px.BSE_OPERATOR(1, 2, 1, 0, epsinv, t, Hqp).configure(opt)
@@ -314,7 +340,7 @@ def test_dynamical_screening_fixtures_gpu(ctx, ints, name, c, nroots, full, tol)
     Hqp = mmio.read_matrix(os.path.join(G, "bse/Hqp.mm"))
     e = np.diag(Hqp).copy()
     t = px.TCMatrix_gwbse(ctx).Initialize(17, 0, 16, 0, 16)
-    t.Fill(T, C, X.pseudo_invsqrt_gwbse(V, Saux))
+    t.Fill(T, C, px.Pseudo_InvSqrt_GWBSE(V, Saux, 5e-7, ctx)[0])
     epsinv = px.setup_screening(t, e, 4, 0, 16)
     opt = px.BSEOperator_Options(4, 0, 0, 0, 16)
     ds = px.DavidsonSolver()
@@ -559,7 +585,7 @@ def ch4_tensor(ctx, ints, mos):
     S, Saux, V, T = ints
     C = mmio.read_matrix(os.path.join(G, mos))
     t = px.TCMatrix_gwbse(ctx).Initialize(17, 0, 16, 0, 16)
-    t.Fill(T, C, X.pseudo_invsqrt_gwbse(V, Saux))
+    t.Fill(T, C, px.Pseudo_InvSqrt_GWBSE(V, Saux, 5e-7, ctx)[0])
     return t
 
 
@@ -596,7 +622,7 @@ def test_ppm_known_answer_gpu(ctx):  # test_ppm.cc:36-90
     e, Cm = np.linalg.eigh(I.core_hamiltonian(xyz, bas))
     S, Saux, V, T = I.ao_integrals(xyz, bas)
     t = px.TCMatrix_gwbse(ctx).Initialize(17, 0, 16, 0, 16)
-    t.Fill(T, Cm, X.pseudo_invsqrt_gwbse(V, Saux))
+    t.Fill(T, Cm, px.Pseudo_InvSqrt_GWBSE(V, Saux, 5e-7, ctx)[0])
     M = t.get_tensor()
     rpa = px.RPA(t)
     rpa.configure(4, 0, 16)

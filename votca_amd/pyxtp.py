@@ -248,6 +248,7 @@ SIGNATURES = {
     "xtpgpu_epsbatch_autotune": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "xtpgpu_epsbatch_result": (C.c_int, [_vp, C.c_int, _dp]),
     "xtpgpu_epsbatch_destroy": (C.c_int, [_vp]),
+    "xtpgpu_aux_pseudo_invsqrt": (C.c_int, [_vp, _dp, _dp, _i64, C.c_double, _dp, C.POINTER(_i64)]),
     "xtpgpu_bse_setup_screening": (C.c_int, [_vp, _vp, _dp, _i64, _i64, _i64, C.c_double, _dp]),
     "xtpgpu_sigma_x": (C.c_int, [_vp, _vp, _i64, _i64, _i64, _i64, _dp]),
     "xtpgpu_rpa_h2p": (C.c_int, [_vp, _vp, _dp, _i64, _i64, _i64, _dp, _dp, _dp]),
@@ -386,6 +387,21 @@ class DeviceBuffer:
             self.free()
         except Exception:
             pass
+
+
+def Pseudo_InvSqrt_GWBSE(V: np.ndarray, S: np.ndarray, etol: float = 5e-7, ctx: Context | None = None):
+    """AOCoulomb::Pseudo_InvSqrt_GWBSE(auxoverlap, etol) (aomatrix.cc:53-86) for the aux Coulomb
+    matrix V and aux overlap S, on the device: returns ((S^-1/2 V S^-1/2)^-1/2 S^-1/2)^T and the
+    number of removed functions (eigenvalues below etol in either step)."""
+    ctx = ctx or default_context()
+    Vf, Sf = _f64(V), _f64(S)
+    n = Vf.shape[0]
+    if Vf.shape != (n, n) or Sf.shape != (n, n):
+        raise ValueError("V and S must be square and of equal size")
+    out = np.zeros((n, n), order="F")
+    rem = _i64(0)
+    _check(lib().xtpgpu_aux_pseudo_invsqrt(ctx.h, _dptr(Vf), _dptr(Sf), n, float(etol), _dptr(out), C.byref(rem)))
+    return out, int(rem.value)
 
 
 class TCMatrix_gwbse:
