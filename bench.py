@@ -755,8 +755,9 @@ def run_adjoint(args):
         """The perturbed problem at step sgn, started from the converged vectors of the nearest
         evaluated step of the same sign (continuation) topped up with the base solve's higher roots,
         or from the base solve's vectors; if the harmonic Ritz step fails from one start (the
-        small nonsymmetric eigenproblem can stall, as the reference's does), from the other."""
-        starts = [warm]
+        small nonsymmetric eigenproblem can stall, as the reference's does), from the next, the
+        last being the default full-BSE guess."""
+        starts = [warm, None]  # the base solve's vectors, then the default full-BSE guess
         if abs(sgn) > 1.0 and sgn / 2.0 in near:
             xn, yn = near[sgn / 2.0]
             starts.insert(0, np.hstack([np.vstack([xn, yn]), warm[:, na:]]))
@@ -766,7 +767,8 @@ def run_adjoint(args):
             except (px.XtpGpuError, RuntimeError) as exc:
                 if k == len(starts) - 1:
                     raise
-                log(f"[adjoint] step {sgn:+.0f}h: {exc}; restarting from the base vectors")
+                log(f"[adjoint] step {sgn:+.0f}h: {exc}; restarting from "
+                    f"{'the default guess' if starts[k + 1] is None else 'the base vectors'}")
 
     for kind, where in todo:
         w, near = {}, {}

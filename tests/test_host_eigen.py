@@ -128,6 +128,20 @@ def test_nonsymmetric_eigenvalues_match_lapack(driver):
         assert np.abs(got - ref).max() < 1e-10 * max(1.0, np.abs(ref).max())
 
 
+def test_nonsymmetric_qr_exceptional_shift_regression(driver):
+    """A 283 x 283 harmonic-Ritz matrix B^-1 T of the full-BSE Davidson (a perturbed solve of the
+    single-segment adjoint checks, captured on the device) with eigenvalues in [0.27, 5.3] and
+    close complex pairs: the Francis iteration stalls on it unless its exceptional shift is centred
+    on the trailing diagonal entry (EISPACK hqr). Eigenvalues against LAPACK's."""
+    A = np.load(os.path.join(ROOT, "tests", "golden", "regress", "harmonic_ritz_283.npy"))
+    ok, wr, wi = run(driver, 2, A)
+    assert ok
+    got = wr + 1j * wi
+    ref = np.linalg.eigvals(A)
+    assert max(np.abs(got - x).min() for x in ref) < 1e-10
+    assert max(np.abs(ref - x).min() for x in got) < 1e-10
+
+
 def test_eigenvector_real_parts_match_lapack(driver):
     """Right eigenvectors for real eigenvalues and the real parts of those for complex ones (unit
     norm, largest-modulus component real: xGEEV's convention), as the harmonic Ritz step keeps

@@ -43,8 +43,39 @@ struct GemmDims {
   int tiles_m, tiles_n;
   int lower;     // 1: only tiles with tile_n <= tile_m are computed (symmetric result)
   int nfast;     // 1: consecutive logical tiles walk along N (reuse of the A row block)
-  int group;     // > 0: grouped order, `group` row blocks x all column blocks at a time
+  int group;     // > 0: grouped order: `group` row blocks x all column blocks at a time (full outputs),
+                 // super-blocks of group x group tiles (lower triangle)
 };
+
+// Lower-triangle tile `tile` (of tiles_m tile rows) in super-blocks of G x G tiles: super-row by
+// super-row, the full blocks left of the diagonal first, then the diagonal block's triangle, so
+// consecutive tiles share G row panels and at most G column panels.
+__device__ __forceinline__ void lower_group_tile(int tile, int tiles_m, int G, int& tm, int& tn) {
+  int cum = 0;
+  for (int R = 0; R * G < tiles_m; ++R) {
+    const int h = min(G, tiles_m - R * G);
+    const int full = R * G * h;
+    const int cnt = full + h * (h + 1) / 2;
+    if (tile < cum + cnt) {
+      const int u = tile - cum;
+      if (u < full) {
+        const int C = u / (G * h), w = u - C * (G * h);
+        tm = R * G + w % h;
+        tn = C * G + w / h;
+      } else {
+        const int v = u - full;
+        int i = (int)((sqrt(8.0 * (double)v + 1.0) - 1.0) * 0.5);
+        while ((i + 1) * (i + 2) / 2 <= v) ++i;
+        while (i * (i + 1) / 2 > v) --i;
+        tm = R * G + i;
+        tn = R * G + (v - i * (i + 1) / 2);
+      }
+      return;
+    }
+    cum += cnt;
+  }
+  tm = tn = 0;  // not reached for tile < tiles_m (tiles_m + 1) / 2
+}
 
 // Block -> (tm, tn). Workgroups are dealt round-robin to the 8 XCDs (block b runs on
 // XCD b % 8); the remap gives every XCD a contiguous range of logical tiles, so tiles that
@@ -54,7 +85,9 @@ __device__ __forceinline__ void tile_of_block(const GemmDims& d, int& tm, int& t
   const int bid = blockIdx.x;
   const int q = nb >> 3, r = nb & 7, x = bid & 7;
   const int tile = x * q + min(x, r) + (bid >> 3);
-  if (d.lower) {
+  if (d.lower && d.group > 0) {
+    lower_group_tile(tile, d.tiles_m, d.group, tm, tn);
+  } else if (d.lower) {
     tm = (int)((sqrt(8.0 * (double)tile + 1.0) - 1.0) * 0.5);
     while ((tm + 1) * (tm + 2) / 2 <= tile) ++tm;
     while (tm * (tm + 1) / 2 > tile) --tm;

@@ -440,10 +440,15 @@ inline bool nonsym_eigvals(int n, const double* A, int lda, double* wr, double* 
     if (++iter > 60 * n) return false;
     // Francis double shift on rows/cols l..hi
     double s, t;
-    if (iter % 11 == 10) {  // exceptional shift
-      double w = std::fabs(H(hi, hi - 1)) + std::fabs(H(hi - 1, hi - 2));
-      s = 1.5 * w;
-      t = w * w;
+    if (iter % 11 == 10) {
+      // exceptional shift (EISPACK hqr): the roots of x^2 - 1.5 w x + w^2 about the trailing
+      // diagonal entry, w the size of the last two subdiagonal entries; centred at zero instead,
+      // it can miss a spectrum far from zero (a 283 x 283 harmonic-Ritz matrix with eigenvalues in
+      // [0.27, 5.3] never deflated)
+      const double w = std::fabs(H(hi, hi - 1)) + std::fabs(H(hi - 1, hi - 2));
+      const double c = H(hi, hi);
+      s = 2.0 * c + 1.5 * w;
+      t = c * c + 1.5 * w * c + w * w;
     } else {
       s = H(hi - 1, hi - 1) + H(hi, hi);
       t = H(hi - 1, hi - 1) * H(hi, hi) - H(hi - 1, hi) * H(hi, hi - 1);

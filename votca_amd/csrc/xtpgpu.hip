@@ -184,6 +184,17 @@ inline int tile_group(const GemmDims& d) {
   return (d.tiles_m >= 64 && d.tiles_n >= 8) ? 8 : 0;
 }
 
+// Super-block size of the lower-triangle tile order (0 = row by row, the default): G x G tiles
+// keep G row and G column panels in flight per XCD instead of one row panel and every column
+// panel. XTPGPU_LOWER_GROUP=G selects it; on the eps builds it measured level with row by row
+// (FP64 51.6 / 51.4, BF16x3 210.8 / 212.1 TFLOP/s at P = 2000).
+inline int lower_tile_group(const GemmDims& d) {
+  const char* e = std::getenv("XTPGPU_LOWER_GROUP");
+  if (e) return std::max(0, std::atoi(e));
+  (void)d;
+  return 0;
+}
+
 template <bool AK, bool BK_, class Epi>
 void gemm64(hipStream_t st, int M, int N, int K, int batch, int ksplit, Opnd<double> A, Opnd<double> B,
             const double* s, long s_bs, bool lower, Epi epi) {
@@ -192,7 +203,7 @@ void gemm64(hipStream_t st, int M, int N, int K, int batch, int ksplit, Opnd<dou
   const long big_tiles = (long)((M + 127) / 128) * ((N + 127) / 128) * batch * std::max(ksplit, 1);
   if (ksplit <= 1 && M >= 256 && N >= 256 && big_tiles >= 512) {
     GemmDims d = make_dims<128, 128>(M, N, std::max(K, 0), batch, ksplit, 16, lower);
-    if (!lower) d.group = tile_group(d);
+    d.group = lower ? lower_tile_group(d) : tile_group(d);
     // registers capped for two waves per SIMD (210 VGPRs, accumulators in VGPRs): 61 against 44
     // TFLOP/s at one wave per SIMD on the Hd stage-1 product; XTPGPU_F64_KERNEL=occ1 selects the latter
     const char* e = std::getenv("XTPGPU_F64_KERNEL");
@@ -243,7 +254,7 @@ void gemm16(hipStream_t st, int M, int N, int K, int batch, int ksplit, Opnd<uns
     if ((choice == 15 || choice == 16) && !s && lds_ok(A, B, K, split) && !(lower && choice == 16)) {
       if (choice == 15) {
         GemmDims d = make_dims<256, 256>(M, N, std::max(K, 0), batch, ksplit, split ? 32 : 64, lower);
-        if (!lower) d.group = tile_group(d);
+        d.group = lower ? lower_tile_group(d) : tile_group(d);
         const char* sc = std::getenv("XTPGPU_LDS_SCHED");
         const int sched = sc ? std::atoi(sc) : 0;
         if constexpr (std::is_same<Epi, EpiStage1>::value) {
@@ -283,7 +294,7 @@ void gemm16(hipStream_t st, int M, int N, int K, int batch, int ksplit, Opnd<uns
     if ((choice == 7 || choice == 8) && bf16_vec(A, B, K, s_bs)) {
       const int bk = split ? 64 : (choice == 8 ? 128 : 64);
       GemmDims d = make_dims<128, 128>(M, N, std::max(K, 0), batch, ksplit, bk, lower);
-      if (!lower) d.group = tile_group(d);
+      d.group = lower ? lower_tile_group(d) : tile_group(d);
       if (bk == 128)
         XG_HIP((launch_gemm_bf16_db<128, AK, BK_>(st, d, A, B, s, s_bs, split, epi)));
       else
@@ -294,13 +305,13 @@ void gemm16(hipStream_t st, int M, int N, int K, int batch, int ksplit, Opnd<uns
     // (148 VGPRs, 37 KB LDS: three workgroups per CU) -- latency hiding by occupancy
     if (choice == 1 && !split && bf16_vec(A, B, K, s_bs)) {
       GemmDims d = make_dims<128, 128>(M, N, std::max(K, 0), batch, ksplit, 64, lower);
-      if (!lower) d.group = tile_group(d);
+      d.group = lower ? lower_tile_group(d) : tile_group(d);
       XG_HIP((launch_gemm_bf16_occ<3, 64, AK, BK_>(st, d, A, B, s, s_bs, split, epi)));
       return;
     }
     if (choice >= 9 && choice <= 13 && bf16_vec(A, B, K, s_bs)) {
       GemmDims d = make_dims<128, 128>(M, N, std::max(K, 0), batch, ksplit, choice >= 12 ? 32 : 64, lower);
-      if (!lower) d.group = tile_group(d);
+      d.group = lower ? lower_tile_group(d) : tile_group(d);
       if (choice == 9) XG_HIP((launch_gemm_bf16_occ<2, 64, AK, BK_>(st, d, A, B, s, s_bs, split, epi)));
       else if (choice == 10) XG_HIP((launch_gemm_bf16_occ<3, 64, AK, BK_>(st, d, A, B, s, s_bs, split, epi)));
       else if (choice == 11) XG_HIP((launch_gemm_bf16_occ<4, 64, AK, BK_>(st, d, A, B, s, s_bs, split, epi)));
@@ -310,7 +321,7 @@ void gemm16(hipStream_t st, int M, int N, int K, int batch, int ksplit, Opnd<uns
     }
     if ((choice == 1 || choice >= 3) && bf16_vec(A, B, K, s_bs)) {
       GemmDims d = make_dims<128, 128>(M, N, std::max(K, 0), batch, ksplit, bf16_wide_kstep(split), lower);
-      if (!lower) d.group = tile_group(d);
+      d.group = lower ? lower_tile_group(d) : tile_group(d);
       XG_HIP((launch_gemm_bf16<128, 128, AK, BK_>(st, d, A, B, s, s_bs, split, epi, true)));
       return;
     }
@@ -786,6 +797,42 @@ bool eps64_scaled() {
   return !(e && std::strcmp(e, "0") == 0);
 }
 
+// K splits of a lower-triangle eps GEMM so that its workgroups fill whole rounds of the resident
+// slots (`resident` = workgroups the kernel keeps on the chip at once): 136 tiles x 12 frequencies at
+// P = 2000 are 3.2 rounds of 512 slots, the last one a fifth full; three splits make 9.6 of 10.
+int eps_ksplit(long units, long resident, long K, int bk) {
+  if (units <= 0 || units >= 8 * resident) return 1;
+  int best = 1;
+  double best_eff = 0.0;
+  for (int ks = 1; ks <= 4; ++ks) {
+    if (ks > 1 && K / ks < 16L * bk) break;
+    const double rounds = (double)(units * ks) / (double)resident;
+    const double eff = rounds / std::ceil(rounds);
+    if (eff > best_eff + 0.02) {
+      best_eff = eff;
+      best = ks;
+    }
+  }
+  return best;
+}
+
+// eps[b] = sum over the K splits of the partials + 1, written to both triangles (the partials hold
+// the lower-triangle tiles: element (m, n), m >= n, of split q at part[((q*nb + b)*P + n)*P + m])
+template <typename T>
+void eps_reduce_splits(hipStream_t st, const T* part, int ks, long nb, long P, double* out) {
+  launch_map(st, (size_t)nb * P * P, [=] __device__(size_t i) {
+    const long b = (long)(i / (size_t)(P * P)), r = (long)(i % (size_t)(P * P));
+    const long n = r / P, m = r % P;
+    if (m < n) return;
+    double acc = 0.0;
+    for (int q = 0; q < ks; ++q) acc += (double)part[(((long)q * nb + b) * P + n) * P + m];
+    const double v = acc + (m == n ? 1.0 : 0.0);
+    double* o = out + b * P * P;
+    o[n * P + m] = v;
+    o[m * P + n] = v;
+  });
+}
+
 void rpa_epsilon_impl(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* energies, int64_t homo, int64_t rpamin,
                       int64_t rpamax, int kind, const double* omega_re, const double* omega_im, int n_omega,
                       double eta, int precision, double* out_dev) {
@@ -849,6 +896,11 @@ void rpa_epsilon_impl(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* energies, in
   });
 
   EpiEpsSym epi{out_dev, P};
+  int cus = 256;
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) cus = prop.multiProcessorCount;
+  }
   // algorithmic FLOPs: n_omega * K * P * (P+1) (symmetric rank-K update); bytes: M read once + eps written
   ProfScope prof(ctx, "rpa_eps_gemm", (double)n_omega * K * P * (P + 1),
                  (double)P * K * (use16 ? (precision == XTPGPU_BF16X3 ? 4 : 2) : 8) + (double)n_omega * P * P * 8);
@@ -877,8 +929,20 @@ void rpa_epsilon_impl(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* energies, in
         for (long f = 0; f < nw_g; ++f) Bs[f * plane + i] = a * dg[f * Kpl + k];
       });
       EpiEpsSym eg{out_dev + w0 * P * P, P};
-      gemm64<true, true>(st, (int)P, (int)P, (int)Kp, (int)nw_g, 1, A, Opnd<double>{Bs, nullptr, Kp, (long)plane},
-                         nullptr, 0, true, eg);
+      const Opnd<double> Bo{Bs, nullptr, Kp, (long)plane};
+      const long tl = (P + 127) / 128;
+      const int ks = P >= 256 ? eps_ksplit(tl * (tl + 1) / 2 * nw_g, 2L * cus, Kp, 16) : 1;
+      if (ks > 1) {  // 128 x 128 two-wave tiles over K splits, FP64 partials, one reduction
+        GemmDims d = make_dims<128, 128>((int)P, (int)P, (int)Kp, (int)nw_g, ks, 16, true);
+        d.group = lower_tile_group(d);
+        DevBuf& pb = ctx->ws[2];
+        pb.ensure((size_t)d.ksplit * nw_g * P * P * 8);
+        XG_HIP((launch_gemm_f64<128, 128, true, true, EpiPartial, 2>(
+            st, d, A, Bo, nullptr, 0, EpiPartial{pb.as<double>(), (int)P, (int)P, (int)nw_g})));
+        eps_reduce_splits(st, pb.as<double>(), d.ksplit, nw_g, P, out_dev + w0 * P * P);
+      } else {
+        gemm64<true, true>(st, (int)P, (int)P, (int)Kp, (int)nw_g, 1, A, Bo, nullptr, 0, true, eg);
+      }
     }
   } else {
     // the weights go into a scaled copy of the transitions, B_w = M diag(d_w) as BF16 hi (+ lo),
@@ -926,8 +990,25 @@ void rpa_epsilon_impl(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* energies, in
         }
       });
       EpiEpsSym eg{out_dev + w0 * P * P, P};
-      gemm16<true, true>(st, (int)P, (int)P, (int)Kp, (int)nw_g, 1, A,
-                         Opnd<unsigned short>{Bh, Bl, Kp, (long)plane}, nullptr, 0, split, true, eg);
+      const Opnd<unsigned short> Bo{Bh, Bl, Kp, (long)plane};
+      const long tl = (P + 127) / 128;
+      // the default kernels keep 2 (split) or 3 (plain) 128 x 128 workgroups per CU resident
+      const int ks = (P >= 256 && bf16_kernel_choice() == 1 && bf16_vec(A, Bo, (int)Kp, 0))
+                         ? eps_ksplit(tl * (tl + 1) / 2 * nw_g, (split ? 2L : 3L) * cus, Kp, 64) : 1;
+      if (ks > 1) {  // the default tiles over K splits, FP32 partials, one reduction
+        GemmDims d = make_dims<128, 128>((int)P, (int)P, (int)Kp, (int)nw_g, ks, 64, true);
+        d.group = lower_tile_group(d);
+        DevBuf& pb = ctx->ws[2];
+        pb.ensure((size_t)d.ksplit * nw_g * P * P * 4);
+        const EpiPart32 ep{pb.as<float>(), P, P, nw_g};
+        if (split)
+          XG_HIP((launch_gemm_bf16<128, 128, true, true>(st, d, A, Bo, nullptr, 0, true, ep, true)));
+        else
+          XG_HIP((launch_gemm_bf16_occ<3, 64, true, true>(st, d, A, Bo, nullptr, 0, false, ep)));
+        eps_reduce_splits(st, pb.as<float>(), d.ksplit, nw_g, P, out_dev + w0 * P * P);
+      } else {
+        gemm16<true, true>(st, (int)P, (int)P, (int)Kp, (int)nw_g, 1, A, Bo, nullptr, 0, split, true, eg);
+      }
     }
   }
   XG_HIP(hipStreamSynchronize(st));
