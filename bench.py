@@ -275,6 +275,10 @@ def run_segments(args):
     # the first wave's solves (all segments submitted at once; set-up excluded), and for the loop
     # also each segment's own solve time.
     nb = min(nseg, args.bse_segments)
+    if nb <= 0:  # eps only
+        print(json.dumps({"workload": f"{nseg} uneven synthetic segments, n_occ U[10,60], n_virt n_occ*U[5,8], "
+                                      f"n_aux 4(n_occ+n_virt)", "n_segments": nseg, **res}))
+        return
     wave = max(1, min(nb, args.bse_wave))
     mixed = args.mixed
     precs = ("fp64", mixed)
