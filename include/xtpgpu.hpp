@@ -225,6 +225,14 @@ class RPA {
     return sol;
   }
 
+  // the inputs of the eps builds, for EpsBatch
+  xtpgpu_mmn mmn() const { return mmn_; }
+  const std::vector<double>& energies() const { return energies_; }
+  int64_t homo() const { return homo_; }
+  int64_t rpamin() const { return rpamin_; }
+  int64_t rpamax() const { return rpamax_; }
+  int64_t auxsize() const { return aux_; }
+
  private:
   // the library reads rpamax - rpamin + 1 input energies
   void check_energies() const {
@@ -242,6 +250,55 @@ class RPA {
   xtpgpu_mmn mmn_;
   int64_t aux_, homo_ = 0, rpamin_ = 0, rpamax_ = 0;
   std::vector<double> energies_;
+};
+
+// eps(i w) of many segments in one hipGraph-replayed persistent launch (xtpgpu_epsbatch_*): the
+// per-segment RPA::calculate_epsilon_i of the jobs ParallelXJobCalc runs (parallelxjobcalc.cc:52-111).
+// The RPA objects (configured, with their input energies) must outlive the batch.
+class EpsBatch {
+ public:
+  EpsBatch(Context& ctx, const std::vector<const RPA*>& rpas, const std::vector<double>& omegas,
+           int precision = XTPGPU_FP64, int grid = 0) {
+    const size_t n = rpas.size();
+    if (n == 0 || omegas.empty()) throw std::runtime_error("EpsBatch: no segments or no frequencies");
+    std::vector<xtpgpu_mmn> t(n);
+    std::vector<const double*> e(n);
+    std::vector<int64_t> homo(n), rmin(n), rmax(n);
+    for (size_t s = 0; s < n; ++s) {
+      const RPA& r = *rpas[s];
+      if ((int64_t)r.energies().size() != r.rpamax() - r.rpamin() + 1)
+        throw std::runtime_error("EpsBatch: RPA input energies do not match rpamax - rpamin + 1");
+      t[s] = r.mmn();
+      e[s] = r.energies().data();
+      homo[s] = r.homo();
+      rmin[s] = r.rpamin();
+      rmax[s] = r.rpamax();
+      aux_.push_back(r.auxsize());
+    }
+    nw_ = (int)omegas.size();
+    check(xtpgpu_epsbatch_create(ctx.get(), (int)n, t.data(), e.data(), homo.data(), rmin.data(), rmax.data(), nw_,
+                                 omegas.data(), precision, grid, &h_));
+  }
+  ~EpsBatch() { xtpgpu_epsbatch_destroy(h_); }
+  EpsBatch(const EpsBatch&) = delete;
+  EpsBatch& operator=(const EpsBatch&) = delete;
+  int autotune() {
+    int g = 0;
+    check(xtpgpu_epsbatch_autotune(h_, &g));
+    return g;
+  }
+  void run(bool use_graph = true) { check(xtpgpu_epsbatch_run(h_, use_graph ? 1 : 0)); }
+  // segment s: P x P x n_omega, column-major
+  std::vector<double> result(int s) const {
+    std::vector<double> out((size_t)aux_.at(s) * aux_.at(s) * nw_);
+    check(xtpgpu_epsbatch_result(h_, s, out.data()));
+    return out;
+  }
+
+ private:
+  xtpgpu_epsbatch h_ = nullptr;
+  int nw_ = 0;
+  std::vector<int64_t> aux_;
 };
 
 // DavidsonSolver (xtp/include/votca/xtp/davidsonsolver.h:47-334) with the device-resident loop:

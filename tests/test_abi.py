@@ -59,7 +59,8 @@ def test_cpp_header_compiles_and_links(tmp_path):
         "    std::vector<std::vector<double>> evb; std::vector<double> secs; int64_t rem = 0;\n"
         "    auto vis = xtpgpu::Pseudo_InvSqrt_GWBSE(c, std::vector<double>(16, 1.0), std::vector<double>(16, 1.0), 4, 5e-7, &rem);\n"
         "    bool ok = ds.solve(H, 2) && ds.solve_full(H, Bop, 2) && ds.solve_batched({H.get()}, 2, 2, evb, &secs);\n"
-        "    return (int)H.rows() + (int)eps.size() + (int)h2p.omega.size() + (int)vis.size() + (ok ? 1 : 0); }\n"
+        "    xtpgpu::EpsBatch eb(c, {&rpa}, {0.5, 1.0}, XTPGPU_BF16X3); eb.autotune(); eb.run(); auto e0 = eb.result(0);\n"
+        "    return (int)H.rows() + (int)eps.size() + (int)h2p.omega.size() + (int)vis.size() + (int)e0.size() + (ok ? 1 : 0); }\n"
         "  return 0; }\n")
     out = tmp_path / "use"
     r = subprocess.run([gxx, "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(out),
