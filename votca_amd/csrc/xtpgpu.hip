@@ -995,7 +995,21 @@ void rpa_epsilon_impl(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* energies, in
       // the default kernels keep 2 (split) or 3 (plain) 128 x 128 workgroups per CU resident
       const int ks = (P >= 256 && bf16_kernel_choice() == 1 && bf16_vec(A, Bo, (int)Kp, 0))
                          ? eps_ksplit(tl * (tl + 1) / 2 * nw_g, (split ? 2L : 3L) * cus, Kp, 64) : 1;
-      if (ks > 1) {  // the default tiles over K splits, FP32 partials, one reduction
+      // split form: 256 x 256 tiles whose operands go straight to LDS (one workgroup per CU), K
+      // split to fill its rounds -- half the panel traffic per FLOP of 128 x 128 tiles, which is
+      // what bounds this GEMM (220 against 206 TFLOP/s at P = 2000, 280 against 229 at P = 6000);
+      // XTPGPU_EPS_BF16_KERNEL=wide keeps the 128 x 128 register-staged tiles
+      const char* ek = std::getenv("XTPGPU_EPS_BF16_KERNEL");
+      const bool wide = ek && std::strcmp(ek, "wide") == 0;
+      if (!wide && bf16_kernel_choice() == 1 && split && P >= 512 && lds_ok(A, Bo, (int)Kp, true)) {
+        const long t2 = (P + 255) / 256;
+        const int k2 = eps_ksplit(t2 * (t2 + 1) / 2 * nw_g, (long)cus, Kp, 32);
+        GemmDims d = make_dims<256, 256>((int)P, (int)P, (int)Kp, (int)nw_g, k2, 32, true);
+        DevBuf& pb = ctx->ws[2];
+        pb.ensure((size_t)d.ksplit * nw_g * P * P * 4);
+        XG_HIP((launch_gemm_bf16_lds<256, 256, 2>(st, d, A, Bo, true, EpiPart32{pb.as<float>(), P, P, nw_g})));
+        eps_reduce_splits(st, pb.as<float>(), d.ksplit, nw_g, P, out_dev + w0 * P * P);
+      } else if (ks > 1) {  // the default tiles over K splits, FP32 partials, one reduction
         GemmDims d = make_dims<128, 128>((int)P, (int)P, (int)Kp, (int)nw_g, ks, 64, true);
         d.group = lower_tile_group(d);
         DevBuf& pb = ctx->ws[2];
