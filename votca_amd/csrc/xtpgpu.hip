@@ -995,13 +995,15 @@ void rpa_epsilon_impl(xtpgpu_ctx ctx, xtpgpu_mmn mmn, const double* energies, in
       // the default kernels keep 2 (split) or 3 (plain) 128 x 128 workgroups per CU resident
       const int ks = (P >= 256 && bf16_kernel_choice() == 1 && bf16_vec(A, Bo, (int)Kp, 0))
                          ? eps_ksplit(tl * (tl + 1) / 2 * nw_g, (split ? 2L : 3L) * cus, Kp, 64) : 1;
-      // split form: 256 x 256 tiles whose operands go straight to LDS (one workgroup per CU), K
-      // split to fill its rounds -- half the panel traffic per FLOP of 128 x 128 tiles, which is
-      // what bounds this GEMM (220 against 206 TFLOP/s at P = 2000, 280 against 229 at P = 6000);
-      // XTPGPU_EPS_BF16_KERNEL=wide keeps the 128 x 128 register-staged tiles
+      // split form from P = 1536: 256 x 256 tiles whose operands go straight to LDS (one
+      // workgroup per CU), K split to fill its rounds -- half the panel traffic per FLOP of
+      // 128 x 128 tiles, which is what bounds this GEMM (220 against 206 TFLOP/s at P = 2000, 280
+      // against 229 at P = 6000); below, the 128 x 128 tiles fill the chip better (a loop of the
+      // 256 uneven segments' builds, P 248-2100, took 532 ms with the large tiles from P = 512
+      // against 480 ms). XTPGPU_EPS_BF16_KERNEL=wide keeps the 128 x 128 tiles throughout.
       const char* ek = std::getenv("XTPGPU_EPS_BF16_KERNEL");
       const bool wide = ek && std::strcmp(ek, "wide") == 0;
-      if (!wide && bf16_kernel_choice() == 1 && split && P >= 512 && lds_ok(A, Bo, (int)Kp, true)) {
+      if (!wide && bf16_kernel_choice() == 1 && split && P >= 1536 && lds_ok(A, Bo, (int)Kp, true)) {
         const long t2 = (P + 255) / 256;
         const int k2 = eps_ksplit(t2 * (t2 + 1) / 2 * nw_g, (long)cus, Kp, 32);
         GemmDims d = make_dims<256, 256>((int)P, (int)P, (int)Kp, (int)nw_g, k2, 32, true);
