@@ -54,6 +54,10 @@ int xtpgpu_device_count(int* count);
 int xtpgpu_ctx_create(int device, unsigned flags, xtpgpu_ctx* out);
 int xtpgpu_ctx_destroy(xtpgpu_ctx ctx);
 int xtpgpu_ctx_synchronize(xtpgpu_ctx ctx);
+/* Free the context's grow-only scratch (kept between calls: the eps builds' scaled transition
+ * copies, up to XTPGPU_EPS_SCALE_BYTES, and their K-split partials); the next call that needs it
+ * allocates it again. No counterpart in the reference (its GPU buffers live per call). */
+int xtpgpu_ctx_release_scratch(xtpgpu_ctx ctx);
 /* Live kernel timing: when enabled, HIP events are recorded on the context's stream around
  * the dominant launches (names: "rpa_eps_gemm", "hd_stage1" / "hd_stage2" for the split BF16x3
  * form, "hd_stage1_bf16" / "hd_stage2_bf16" for plain BF16, "hd_stage1_fp64" / "hd_stage2_fp64"

@@ -36,6 +36,8 @@ class Context {
   Context(const Context&) = delete;
   Context& operator=(const Context&) = delete;
   xtpgpu_ctx get() const { return h_; }
+  // return the grow-only scratch the eps builds keep between calls to the device
+  void release_scratch() { check(xtpgpu_ctx_release_scratch(h_)); }
 
  private:
   xtpgpu_ctx h_ = nullptr;

@@ -60,6 +60,7 @@ int main(int argc, char** argv) {
   batch.autotune();
   batch.run(true);
   batch.run(true);
+  ctx.release_scratch();  // the per-segment builds below allocate the kept scratch again
   double eps_rel = 0.0;
   for (int64_t s = 0; s < nseg; ++s) {
     const std::vector<double> b = batch.result((int)s);

@@ -184,6 +184,24 @@ def test_rpa_fp64_scaled_copies(ctx, monkeypatch, budget):
         assert np.array_equal(g[:, :, i], g[:, :, i].T)
 
 
+def test_release_scratch_then_rebuild(ctx):
+    """The context's kept eps scratch freed between builds: the builds after it allocate it again
+    and give the same bits as before."""
+    rng = np.random.default_rng(8)
+    o, v, P = 6, 50, 120
+    M = rng.normal(scale=0.3, size=(o + v, o + v, P))
+    e = np.concatenate([np.sort(rng.uniform(-1, -0.2, o)), np.sort(rng.uniform(0.3, 1.5, v))])
+    rpa = px.RPA(dev_mmn(ctx, M))
+    rpa.configure(o - 1, 0, o + v - 1)
+    rpa.setRPAInputEnergies(e)
+    freqs = [0.0, 0.7]
+    before = {p: rpa.calculate_epsilon(px.EPS_IMAG, freqs, p) for p in ("fp64", "bf16x3")}
+    for _ in range(2):
+        ctx.release_scratch()
+        for p, g in before.items():
+            assert np.array_equal(rpa.calculate_epsilon(px.EPS_IMAG, freqs, p), g), p
+
+
 @pytest.mark.parametrize("kern", ["staged32", "staged64"])
 def test_rpa_bf16x3_kernel_variants(ctx, kern, monkeypatch):
     """eps(i w) through both register-staged BF16 kernels (K step 32 and 64, on the scaled

@@ -439,6 +439,17 @@ extern "C" int xtpgpu_ctx_synchronize(xtpgpu_ctx ctx) {
   });
 }
 
+// The context's grow-only scratch (the ε builds' scaled transition copies and K-split partials, up
+// to the scale budget plus the partials) is kept between calls; this returns it to the device.
+extern "C" int xtpgpu_ctx_release_scratch(xtpgpu_ctx ctx) {
+  return guarded([&] {
+    XG_REQUIRE(ctx, XTPGPU_ERR_ARG, "null ctx");
+    XG_HIP(hipSetDevice(ctx->device));
+    XG_HIP(hipStreamSynchronize(ctx->stream));
+    for (auto& b : ctx->ws) b.release();
+  });
+}
+
 extern "C" int xtpgpu_ctx_set_profiling(xtpgpu_ctx ctx, int enable) {
   return guarded([&] {
     XG_REQUIRE(ctx, XTPGPU_ERR_ARG, "null ctx");

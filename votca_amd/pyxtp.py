@@ -180,6 +180,7 @@ SIGNATURES = {
     "xtpgpu_ctx_create": (C.c_int, [C.c_int, C.c_uint, C.POINTER(_vp)]),
     "xtpgpu_ctx_destroy": (C.c_int, [_vp]),
     "xtpgpu_ctx_synchronize": (C.c_int, [_vp]),
+    "xtpgpu_ctx_release_scratch": (C.c_int, [_vp]),
     "xtpgpu_ctx_set_profiling": (C.c_int, [_vp, C.c_int]),
     "xtpgpu_ctx_timer": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(_i64),
                                    C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -320,6 +321,10 @@ class Context:
 
     def synchronize(self):
         _check(lib().xtpgpu_ctx_synchronize(self.h))
+
+    def release_scratch(self):
+        """Free the grow-only scratch the eps builds keep between calls (scaled copies, partials)."""
+        _check(lib().xtpgpu_ctx_release_scratch(self.h))
 
     def set_profiling(self, on: bool = True):
         _check(lib().xtpgpu_ctx_set_profiling(self.h, 1 if on else 0))
