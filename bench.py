@@ -270,7 +270,8 @@ def run_segments(args):
     # whose operators are resident together: each wave is set up (screening on the device, operator
     # packed, the tensor freed once packed), then solved one segment after another with the
     # single-segment entry point and all together with the batched entry point (`--bse-workers`
-    # worker streams, dealt largest first and smallest first), in FP64 and in the mixed form with FP64
+    # worker streams, dealt largest first and smallest first, and largest first with the workers'
+    # products issued as grouped launches), in FP64 and in the mixed form with FP64
     # refinement to the same 1e-5 residual. Latency: each segment's completion time from the start of
     # the first wave's solves (all segments submitted at once; set-up excluded), and for the loop
     # also each segment's own solve time.
@@ -282,7 +283,8 @@ def run_segments(args):
     wave = max(1, min(nb, args.bse_wave))
     mixed = args.mixed
     precs = ("fp64", mixed)
-    orders = ("largest_first", "smallest_first")
+    # "grouped": largest first, the workers' products issued together (one launch per step)
+    orders = ("largest_first", "smallest_first", "grouped")
 
     def solver(p):
         ds = px.DavidsonSolver()
@@ -335,7 +337,9 @@ def run_segments(args):
             for order in orders:
                 b = accb[(p, order)]
                 t0 = time.perf_counter()
-                ev_b, _, infos, sec, status = ds.solve_batched(ops, 8, workers=args.bse_workers, order=order)
+                ev_b, _, infos, sec, status = ds.solve_batched(
+                    ops, 8, workers=args.bse_workers, order="largest_first" if order == "grouped" else order,
+                    grouped=order == "grouped")
                 dt = time.perf_counter() - t0
                 if not ds.converged():
                     raise RuntimeError(f"batched segment BSE ({p}) did not converge: status {status.tolist()}")
@@ -361,7 +365,7 @@ def run_segments(args):
             f"{pct(a['own'], 50):.0f}/{pct(a['own'], 99):.0f} ms)")
         for order in orders:
             b = accb[(p, order)]
-            key = f"bse_{tag}_batched" + ("" if order == "largest_first" else "_smallest_first")
+            key = f"bse_{tag}_batched" + ("" if order == "largest_first" else "_" + order)
             res.update({f"{key}_s": b["t"], f"{key}_segments_per_s": nb / b["t"],
                         f"{key}_completion_ms_p50": pct(b["done"], 50),
                         f"{key}_completion_ms_p99": pct(b["done"], 99),

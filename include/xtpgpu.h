@@ -445,6 +445,11 @@ int xtpgpu_davidson_solve_ham_mf(xtpgpu_mfop R, xtpgpu_mfop C, int64_t neigen,
  * return code. Returns the first segment error, else XTPGPU_ERR_NOT_CONVERGED when a segment did
  * not converge (its roots zeroed as in xtpgpu_davidson_solve), else XTPGPU_OK. */
 enum { XTPGPU_BATCH_LARGEST_FIRST = 0, XTPGPU_BATCH_SMALLEST_FIRST = 1, XTPGPU_BATCH_AS_GIVEN = 2 };
+/* OR-ed into `order`: the workers' operator products meet at a rendezvous and are issued together,
+ * one launch per contraction step for the segments posted (half the workers still solving at a
+ * time; XTPGPU_BATCH_SUBGROUPS=1 waits for all of them). The Davidson iterations stay per segment;
+ * products of streamed-FP64, Hd2 or non-diagonal-Hqp operators run one by one. */
+enum { XTPGPU_BATCH_GROUPED = 0x100 };
 int xtpgpu_davidson_solve_batched(int64_t nops, const xtpgpu_bseop* ops, int64_t neigen,
                                   const xtpgpu_davidson_opts* opts, int32_t workers, int32_t order,
                                   int64_t scratch_bytes, double* evals, double* const* evecs,

@@ -371,6 +371,7 @@ class DavidsonSolver {
   // parallelxjobcalc.cc:52-111) on `workers` streams in `order`: ops are BSEOperator::get()
   // handles; evals gets one row of neigen per operator, seconds (optional) each segment's completion
   // time from the call's start. Returns false when a segment did not converge.
+  // order: XTPGPU_BATCH_*, optionally | XTPGPU_BATCH_GROUPED (products issued together)
   bool solve_batched(const std::vector<xtpgpu_bseop>& ops, int64_t neigen, int workers,
                      std::vector<std::vector<double>>& evals, std::vector<double>* seconds = nullptr,
                      int order = XTPGPU_BATCH_LARGEST_FIRST) {

@@ -303,9 +303,9 @@ def test_segments_tda_batched(ctx):
     """The same law's 16 segments solved together (xtpgpu_davidson_solve_batched: worker threads
     with their own streams, largest segment first) in FP64 and in the mixed BF16x3 form: every
     segment's 8 roots against dense eigh (1e-7 Ha) and against its own single-segment solve (1e-10
-    Ha), with one and four workers dealt largest first and three dealt smallest first; eigenvectors
-    against the single solve's up to sign; a scratch region below one product's need is refused
-    before any solve."""
+    Ha), with one and four workers dealt largest first and three dealt smallest first, and with the
+    workers' products issued together (grouped launches); eigenvectors against the single solve's
+    up to sign; a scratch region below one product's need is refused before any solve."""
     segs = _segments(33, 16)
     ops, ts, dense = [], [], []
     for s, (o, v, P) in enumerate(segs):
@@ -330,8 +330,12 @@ def test_segments_tda_batched(ctx):
         if prec != "fp64":
             ds.set_precision(prec, refine_tolerance=1e-8, refine_max=8)
         single = [ds.solve(op, 8, want_vectors=True) for op in ops]
-        for workers, order in ((1, "largest_first"), (4, "largest_first"), (3, "smallest_first")):
-            ev, vecs, infos, sec, status = ds.solve_batched(ops, 8, workers=workers, want_vectors=True, order=order)
+        for workers, order, grouped in ((1, "largest_first", False), (4, "largest_first", False),
+                                        (3, "smallest_first", False), (4, "largest_first", True),
+                                        (3, "smallest_first", True)):
+            print(f"[batched] {prec} workers={workers} {order} grouped={grouped}", flush=True)
+            ev, vecs, infos, sec, status = ds.solve_batched(ops, 8, workers=workers, want_vectors=True, order=order,
+                                                            grouped=grouped)
             assert ds.converged() and (status == 0).all(), (prec, workers, status)
             assert (sec > 0).all() and len(infos) == len(ops)
             for s in range(len(ops)):
@@ -346,12 +350,14 @@ def test_segments_tda_batched(ctx):
     # error); an operator without the asked precision fails its segment, which the error names
     ds = px.DavidsonSolver()
     ds.set_iter_max(2)
-    ev, _, infos, sec, status = ds.solve_batched(ops[:5], 8, workers=3)
-    assert not ds.converged() and (status == -4).all() and all(i.converged == 0 for i in infos)
+    for grouped in (False, True):
+        ev, _, infos, sec, status = ds.solve_batched(ops[:5], 8, workers=3, grouped=grouped)
+        assert not ds.converged() and (status == -4).all() and all(i.converged == 0 for i in infos)
     ds = px.DavidsonSolver()
     ds.set_precision("bf16x3", refine_tolerance=1e-6)
-    with pytest.raises(px.XtpGpuError, match="segment 2: .*without this precision"):
-        ds.solve_batched([ops[1], ops[2], fp64_only], 8, workers=2)
+    for grouped in (False, True):
+        with pytest.raises(px.XtpGpuError, match="segment 2: .*without this precision"):
+            ds.solve_batched([ops[1], ops[2], fp64_only], 8, workers=2, grouped=grouped)
     fp64_only.close()
     for op, t in zip(ops, ts):
         op.close()

@@ -202,6 +202,46 @@ def test_release_scratch_then_rebuild(ctx):
             assert np.array_equal(rpa.calculate_epsilon(px.EPS_IMAG, freqs, p), g), p
 
 
+@pytest.mark.parametrize("c", [(1, 2, 1, 0), (1, 0, 1, 0), (0, 2, 0, 0), (0, 0, 1, 0)])
+@pytest.mark.parametrize("chunks", [False, True])
+def test_grouped_products_match_single(ctx, c, chunks, monkeypatch):
+    """The grouped launches of the many-segment solve on one operator (xtpgpu_bse_matmul with
+    XTPGPU_MATMUL_GROUPED=1) against the single-operator product and, in FP64, the oracle's dense
+    matrix (bse_operator.cc:39-119), with one stage-1 aux chunk and with several (uneven last)."""
+    rng = np.random.default_rng(12)
+    o, v, P = 9, 70, 150
+    n = o + v
+    t = px.TCMatrix_gwbse(ctx).Initialize(P, 0, n - 1, 0, n - 1)
+    t.fill_synthetic(12, 0.05, 3.0)
+    e = X.synthetic_energies(12, o, v)
+    eps = rng.uniform(0.3, 1.0, P)
+    opt = px.BSEOperator_Options(o - 1, 0, 0, 0, n - 1)
+    op = px.BSE_OPERATOR(*c, eps, t, np.diag(e), precisions=("fp64", "bf16", "bf16x3")).configure(opt)
+    Xm = rng.normal(size=(op.size, 7))
+    H = X.bse_dense(*c, eps, t.get_tensor(), np.diag(e), o - 1, 0, 0, n - 1)
+    if chunks:  # FP64 stage-1 block per aux function: 70 x 7 x 16 x 8 bytes; 41 per chunk
+        monkeypatch.setenv("XTPGPU_HD_CHUNK_BYTES", str(41 * 70 * 7 * 16 * 8))
+    for prec in ("fp64", "bf16x3", "bf16"):
+        monkeypatch.delenv("XTPGPU_MATMUL_GROUPED", raising=False)
+        y1 = op.matmul(Xm, prec)
+        monkeypatch.setenv("XTPGPU_MATMUL_GROUPED", "1")
+        yg = op.matmul(Xm, prec)
+        scale = np.abs(y1).max()
+        if prec == "fp64":
+            assert np.abs(yg - y1).max() <= 1e-13 * scale, (c, chunks)
+            assert np.abs(yg - H @ Xm).max() <= 1e-12 * scale, (c, chunks)
+        else:  # the same BF16 operands; only the FP32 accumulation of Hd stage 2 is split differently
+            assert np.abs(yg - y1).max() <= 5e-5 * scale, (c, chunks, prec, np.abs(yg - y1).max() / scale)
+    if c[0]:  # a non-diagonal Hqp is not grouped: refused here, run one by one in the batched solve
+        Hq = np.diag(e) + 1e-3 * np.ones((n, n))
+        opn = px.BSE_OPERATOR(*c, eps, t, Hq, precisions=("fp64",)).configure(opt)
+        with pytest.raises(px.XtpGpuError, match="not grouped"):
+            opn.matmul(Xm)
+        opn.close()
+    op.close()
+    t.close()
+
+
 @pytest.mark.parametrize("kern", ["staged32", "staged64"])
 def test_rpa_bf16x3_kernel_variants(ctx, kern, monkeypatch):
     """eps(i w) through both register-staged BF16 kernels (K step 32 and 64, on the scaled

@@ -13,9 +13,11 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <tuple>
 #include <type_traits>
@@ -1167,7 +1169,8 @@ MatmulPlan plan_matmul(const xtpgpu_bseop_s* op, int precision, long k) {
     if (split) pl.xl = (size_t)Np * k * 2;
   }
   if (op->cd || op->cd2) pl.per_p = (size_t)std::max(ct, vt) * k * op->vtp * (m16 ? 2 * (split ? 2 : 1) : 8);
-  if (op->cd && m16) pl.part = (size_t)4 * k * ct * vt * 4;  // Hd stage-2 FP32 partials, at most 4 K splits
+  // Hd stage-2 partials, at most 4 K splits (FP32; FP64 in the grouped products of the batched solve)
+  if (op->cd) pl.part = (size_t)4 * k * ct * vt * (m16 ? 4 : 8);
   if (op->cx) {
     const int tiles = (int)(((P + 63) / 64) * ((k + 63) / 64));
     const GemmDims dd = make_dims<64, 64>((int)P, (int)k, (int)Np, 1, pick_ksplit(tiles, Np, 16), 16, false);
@@ -1429,6 +1432,8 @@ void bse_matmul_impl(xtpgpu_bseop op, int precision, const double* X, long ldx, 
 }
 
 }  // namespace
+
+#include "grouped.inc"
 
 // BSE::AdjustHqpSize (bse.cc:147-186): host-side, no device work. The BSE window [vmin, cmax]
 // (vtotal = homo - vmin + 1 valence, ctotal = cmax - homo conduction levels) takes the GW-window
@@ -2012,7 +2017,10 @@ extern "C" int xtpgpu_bse_matmul(xtpgpu_bseop op, int precision, const double* X
     const size_t n = (size_t)op->N * k;
     DevBuf dx(n * 8), dy(n * 8);
     XG_HIP(hipMemcpy2DAsync(dx.p, op->N * 8, X, ldx * 8, op->N * 8, k, hipMemcpyHostToDevice, st));
-    bse_matmul_impl(op, precision, dx.as<double>(), op->N, k, dy.as<double>(), op->N);
+    if (grouped_matmul_env())
+      grouped_matmul_one(op, precision, dx.as<double>(), k, dy.as<double>(), st);
+    else
+      bse_matmul_impl(op, precision, dx.as<double>(), op->N, k, dy.as<double>(), op->N);
     XG_HIP(hipMemcpy2DAsync(Y, ldy * 8, dy.p, op->N * 8, op->N * 8, k, hipMemcpyDeviceToHost, st));
     XG_HIP(hipStreamSynchronize(st));
   });

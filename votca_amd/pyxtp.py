@@ -975,10 +975,11 @@ class DavidsonSolver:
     _ORDER = {"largest_first": 0, "smallest_first": 1, "as_given": 2}
 
     def solve_batched(self, ops, neigen: int, workers: int = 4, want_vectors=False, scratch_bytes: int = 0,
-                      order: str = "largest_first"):
+                      order: str = "largest_first", grouped: bool = False):
         """Many segments' TDA problems (the per-segment jobs of ParallelXJobCalc,
         parallelxjobcalc.cc:52-111) with this solver's options, on `workers` streams, segments dealt
-        largest operator first, smallest first or as given. Returns (evals (nops x neigen),
+        largest operator first, smallest first or as given. grouped: the workers' operator products
+        are issued together, one launch per contraction step for all segments in flight. Returns (evals (nops x neigen),
         vectors (list, or None), infos, seconds: each segment's completion time from the call's
         start, status: per-segment return codes)."""
         if order not in self._ORDER:
@@ -994,7 +995,8 @@ class DavidsonSolver:
         infos = (DavidsonInfo * n)()
         sec = np.zeros(n)
         status = (C.c_int32 * n)()
-        rc = lib().xtpgpu_davidson_solve_batched(n, hs, neigen, C.byref(self.opts), int(workers), self._ORDER[order],
+        rc = lib().xtpgpu_davidson_solve_batched(n, hs, neigen, C.byref(self.opts), int(workers),
+                                                 self._ORDER[order] | (0x100 if grouped else 0),
                                                  int(scratch_bytes), _dptr(ev), vp, infos, _dptr(sec), status)
         self._converged = rc == 0
         if rc not in (0, -4):
