@@ -30,6 +30,15 @@ def test_binding_covers_header():
         set(declared_symbols()) ^ set(pyxtp.SIGNATURES)
 
 
+def test_batch_enums_match_header():
+    """The batched solve's order values and grouped flag in the binding are the header's."""
+    h = open(os.path.join(ROOT, "include", "xtpgpu.h")).read()
+    vals = {m.group(1): int(m.group(2), 0) for m in re.finditer(r"XTPGPU_BATCH_(\w+) = (0x[0-9a-fA-F]+|\d+)", h)}
+    ds = pyxtp.DavidsonSolver
+    assert vals == {"LARGEST_FIRST": ds._ORDER["largest_first"], "SMALLEST_FIRST": ds._ORDER["smallest_first"],
+                    "AS_GIVEN": ds._ORDER["as_given"], "GROUPED": ds._GROUPED}, vals
+
+
 def test_binding_loads():
     L = pyxtp.lib()
     assert L.xtpgpu_last_error() is not None

@@ -973,6 +973,7 @@ class DavidsonSolver:
         return self._finish(rc, ev, vec)
 
     _ORDER = {"largest_first": 0, "smallest_first": 1, "as_given": 2}
+    _GROUPED = 0x100  # XTPGPU_BATCH_GROUPED
 
     def solve_batched(self, ops, neigen: int, workers: int = 4, want_vectors=False, scratch_bytes: int = 0,
                       order: str = "largest_first", grouped: bool = False):
@@ -996,7 +997,7 @@ class DavidsonSolver:
         sec = np.zeros(n)
         status = (C.c_int32 * n)()
         rc = lib().xtpgpu_davidson_solve_batched(n, hs, neigen, C.byref(self.opts), int(workers),
-                                                 self._ORDER[order] | (0x100 if grouped else 0),
+                                                 self._ORDER[order] | (self._GROUPED if grouped else 0),
                                                  int(scratch_bytes), _dptr(ev), vp, infos, _dptr(sec), status)
         self._converged = rc == 0
         if rc not in (0, -4):
